@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""TPC-H Q1 scan -> filter -> group-aggregate on one MI355X.
+
+A step is one pass of Q1 over the whole device-resident table (600M rows, compressed
+layout: 9.375 B/row), launched through the C ABI's q1_enqueue (the
+drop-in kernel launch) into one device aggregate table.  Timed region: K back-to-back
+passes between two synchronisations; value = rows * K / seconds.  The kernel's average
+duration comes from HIP events recorded on the library's stream around the K launches.
+
+Prints one JSON line (see the fields below).  --gpus N > 1 exits: this path runs on one
+GPU here (a multi-GPU run would shard by row range and exchange a 384-byte table).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rows/s + % HBM roofline at 600M rows; batched req/s and p99 latency"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=600_000_000)
+    ap.add_argument("--layout", default="compressed", choices=["compressed", "uncompressed", "reference"])
+    ap.add_argument("--placement", default="auto")
+    ap.add_argument("--threads-per-block", type=int, default=0)
+    ap.add_argument("--tuples-per-thread", type=int, default=0)
+    ap.add_argument("--cpu-rows", type=int, default=60_000_000, help="CPU baseline sample (reference layout)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-passes", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(path: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled)."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(q1, rows: int, threads: int, passes: int):
+    """The oracle's morsel-parallel restatement of the reference CPU kernel KernelNaive
+    (cpu/kernel_naive.hpp) on a bounded sample of the same generator's rows in the
+    reference layout; the first pass is cold and dropped (cpu/q1.cpp:268-299)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from checker import Oracle
+
+    o = Oracle()
+    with q1.Context(0) as c:
+        t = c.generate(q1.REFERENCE, 0x51C00000, rows)
+        cols = t.download()
+        t.free()
+    times = []
+    for _ in range(passes):
+        t0 = time.perf_counter()
+        o.naive_parallel(cols, 0, rows, 729999, threads)
+        times.append(time.perf_counter() - t0)
+    hot = times[1:] if len(times) > 1 else times
+    sec = sum(hot) / len(hot)
+    return {"value": rows / sec, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} rows, reference layout (38 B/row), oracle KernelNaive restatement, "
+                      f"morsel-parallel, mean of {len(hot)} hot passes",
+            "ms_per_pass": sec * 1e3}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from tpchq01_gpu_amd import q1
+
+    layout = {"compressed": q1.COMPRESSED, "uncompressed": q1.UNCOMPRESSED, "reference": q1.REFERENCE}[args.layout]
+    placement = q1.PLACEMENTS[args.placement]
+    rows = args.rows
+    ctx = q1.Context(0)
+    info = ctx.device_info()
+    log(f"bench: device {info.name.decode()} {info.arch.decode()} CUs={info.compute_units}")
+    log(f"bench: generating {rows} rows ({q1.LAYOUT_NAMES[layout]}) on the device")
+    table = ctx.generate(layout, 0x51C00002, rows)
+    if placement == q1.AUTO:
+        # the autotuning table's best placement for this layout
+        ctx.run(table, 0, min(rows, 1 << 20), 729999, q1.AUTO)
+        placement = ctx.last_timing().placement
+    tuned = ctx.tuning_get(layout, placement)
+    launch = q1.make_launch(args.threads_per_block or tuned.threads_per_block,
+                            args.tuples_per_thread or tuned.tuples_per_thread, 0,
+                            q1.LAUNCH_ASSUME_TPCH_DOMAIN)
+    log(f"bench: placement {q1.PLACEMENT_NAMES[placement]} tpb={launch.threads_per_block} "
+        f"tpt={launch.tuples_per_thread}")
+
+    # one reference pass through q1_run (fully reduced, host result)
+    single = ctx.run_raw(table, 0, rows, 729999, placement, launch)
+    single_ms = ctx.last_timing().kernel_ms
+    log(f"bench: single q1_run kernel {single_ms:.3f} ms")
+
+    dev_cols = table.device_columns()
+    agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.ExternalStream(ctx.stream())
+    p_agg = C.c_void_p(agg.data_ptr())
+
+    def enqueue():
+        ctx.check(q1.lib.q1_enqueue(ctx.handle, placement, layout, C.byref(launch), p_agg, C.byref(dev_cols),
+                                    0, rows, 729999, None), "q1_enqueue")
+
+    for _ in range(args.warmup):
+        enqueue()
+    ctx.synchronize()
+    with torch.cuda.stream(stream):
+        agg.zero_()
+    torch.cuda.synchronize()
+    ctx.synchronize()
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        enqueue()
+    ev1.record(stream)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+
+    # K passes accumulate K times the single-pass table: a size-independent exactness check
+    host = q1.DeviceAggregates()
+    C.memmove(C.byref(host), agg.cpu().numpy().ctypes.data, C.sizeof(host))
+    res = q1.Result()
+    q1.lib.q1_aggregates_to_result(C.byref(host), C.byref(res))
+    got, one = q1.result_to_list(res), q1.result_to_list(single)
+    k = args.steps
+    ok = all(g.count == k * s.count and g.sum_charge == k * s.sum_charge and
+             g.sum_disc_price == k * s.sum_disc_price and g.sum_qty == k * s.sum_qty for g, s in zip(got, one))
+    if not ok:
+        log("bench: ERROR accumulated passes differ from K x single pass")
+        sys.exit(1)
+
+    algo_bytes = rows * q1.BITS_PER_ROW[layout] // 8
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(args.traffic_file) if layout == q1.COMPRESSED else None
+    out = {
+        "metric": METRIC,
+        "value": rows * args.steps / elapsed,
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded generator, BASELINE config distributions)",
+        "config": {"workload": f"TPC-H Q1, {rows} device-resident rows, {q1.LAYOUT_NAMES[layout]} layout",
+                   "rows": rows, "layout": q1.LAYOUT_NAMES[layout],
+                   "placement": q1.PLACEMENT_NAMES[placement],
+                   "threads_per_block": launch.threads_per_block,
+                   "tuples_per_thread": launch.tuples_per_thread,
+                   "bytes_per_row": q1.BITS_PER_ROW[layout] / 8},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": algo_bytes},
+    }
+    if not args.no_cpu:
+        log(f"bench: CPU baseline on {args.cpu_rows} rows x {args.cpu_passes} passes, {args.cpu_threads} threads")
+        out["cpu_baseline"] = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)
+    else:
+        out["cpu_baseline"] = None
+    table.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

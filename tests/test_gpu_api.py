@@ -1,0 +1,147 @@
+"""The drop-in kernel launch (q1_enqueue), wrapped device columns, batching and
+errors, through the C ABI on the GPU."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from tpchq01_gpu_amd import q1
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x51C00003
+
+
+def as_dicts(groups):
+    return [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
+                 sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in groups]
+
+
+def note(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def device_aggregates_to_result(t):
+    host = q1.DeviceAggregates()
+    C.memmove(C.byref(host), t.cpu().numpy().ctypes.data, C.sizeof(host))
+    r = q1.Result()
+    q1.lib.q1_aggregates_to_result(C.byref(host), C.byref(r))
+    return q1.result_to_list(r)
+
+
+def test_enqueue_composes_by_addition(ctx, oracle):
+    """Launches over disjoint row ranges add into one caller-zeroed device table, like the
+    reference's per-chunk launches (src/main.cu:553-558, 657-718)."""
+    import torch
+    rows = 2_000_011
+    ref = q1.generate_host(q1.REFERENCE, SEED, rows)
+    comp = q1.generate_host(q1.COMPRESSED, SEED, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    cols = t.device_columns()
+    for p in range(5):
+        agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        cuts = [0, 17, 1 << 20, (1 << 20) + 33, rows]
+        for b, e in zip(cuts, cuts[1:]):
+            l = q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN)
+            ctx.check(q1.lib.q1_enqueue(ctx.handle, p, q1.COMPRESSED, C.byref(l), C.c_void_p(agg.data_ptr()),
+                                        C.byref(cols), b, e - b, 729999, None), "q1_enqueue")
+        ctx.synchronize()
+        assert as_dicts(device_aggregates_to_result(agg)) == want, p
+        # same without the domain assertion (64-bit kernels)
+        agg.zero_()
+        torch.cuda.synchronize()
+        ctx.check(q1.lib.q1_enqueue(ctx.handle, p, q1.COMPRESSED, None, C.c_void_p(agg.data_ptr()),
+                                    C.byref(cols), 0, rows, 729999, None), "q1_enqueue")
+        ctx.synchronize()
+        assert as_dicts(device_aggregates_to_result(agg)) == want, p
+
+
+def test_wrap_torch_columns(ctx, oracle):
+    import torch
+    rows = 123_457
+    ref = q1.generate_host(q1.REFERENCE, SEED + 1, rows)
+    unc = q1.generate_host(q1.UNCOMPRESSED, SEED + 1, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729906)
+    dev = {k: torch.from_numpy(v.copy()).to("cuda") for k, v in unc.items()}
+    torch.cuda.synchronize()
+    c = q1.Columns()
+    for k, v in dev.items():
+        setattr(c, k, v.data_ptr())
+    t = ctx.wrap(q1.UNCOMPRESSED, c, rows)
+    for p in range(5):
+        assert as_dicts(ctx.run(t, 0, rows, 729906, p)) == want
+    t.free()
+    # misaligned column pointer is refused
+    c.quantity = dev["quantity"].data_ptr() + 4
+    with pytest.raises(q1.Q1Error):
+        ctx.wrap(q1.UNCOMPRESSED, c, rows - 1)
+
+
+def test_errors(ctx):
+    t = ctx.generate(q1.COMPRESSED, SEED, 1000)
+    with pytest.raises(q1.Q1Error):
+        ctx.run(t, 900, 200)                      # range past the end
+    with pytest.raises(q1.Q1Error):
+        ctx.run(t, 0, 1000, launch=q1.make_launch(100, 1024))   # not a multiple of 64
+    with pytest.raises(q1.Q1Error):
+        ctx.run(t, 0, 1000, launch=q1.make_launch(256, 1000))   # not a multiple of 16
+    with pytest.raises(q1.Q1Error):
+        ctx.run(t, 0, 1000, launch=q1.make_launch(256, 1 << 15))  # above 16384
+    with pytest.raises(q1.Q1Error):
+        ctx.build_filter(ctx.generate(q1.REFERENCE, SEED, 10), 729999)
+    assert ctx.run(t, 1000, 0)[0].count == 0     # empty range
+
+
+def test_device_info(ctx):
+    d = ctx.device_info()
+    note("device", d.name, d.arch, d.compute_units, d.total_memory)
+    assert d.compute_units > 0 and d.wavefront_size == 64
+    assert b"gfx950" in d.arch
+
+
+def make_requests(ctx, tables, rng, n):
+    reqs, meta = [], []
+    for i in range(n):
+        t = tables[int(rng.integers(0, len(tables)))]
+        size = int(np.exp(rng.uniform(np.log(1), np.log(t.rows))))
+        begin = int(rng.integers(0, t.rows - size + 1))
+        cutoff = int(rng.integers(729906, 730089))
+        reqs.append(q1.Request(t.handle, begin, size, cutoff, 0))
+        meta.append((t, begin, size, cutoff))
+    return reqs, meta
+
+
+def test_batch_matches_single_runs_across_graph_replays(ctx):
+    rng = np.random.default_rng(1)
+    tables = [ctx.generate(q1.COMPRESSED, SEED + k, rows) for k, rows in enumerate([1000, 65_536, 1_000_003, 3_000_000])]
+    for rep, n in enumerate([1, 7, 16, 16, 100, 256, 3, 256]):
+        reqs, meta = make_requests(ctx, tables, rng, n)
+        arr = (q1.Request * n)(*reqs)
+        outs = (q1.Result * n)()
+        note(f"case batch replay {rep} n={n}")
+        ctx.check(q1.lib.q1_run_batch(ctx.handle, arr, n, None, outs), "q1_run_batch")
+        # eager work between replays, then every request alone
+        for i, (t, b, s, c) in enumerate(meta):
+            assert q1.result_to_list(outs[i]) == ctx.run(t, b, s, c, q1.IN_REGISTERS), (rep, i)
+
+
+def test_batch_pushdown_and_mixed_layouts(ctx, oracle):
+    rows = 200_000
+    comp = ctx.generate(q1.COMPRESSED, SEED + 5, rows)
+    pd = ctx.build_filter(comp, 729950)
+    ref = ctx.generate(q1.REFERENCE, SEED + 5, rows)
+    reqs = [q1.Request(pd.handle, 5, rows - 10, 0, 0), q1.Request(pd.handle, 0, 77, 0, 0)]
+    outs = (q1.Result * 2)()
+    ctx.check(q1.lib.q1_run_batch(ctx.handle, (q1.Request * 2)(*reqs), 2, None, outs), "q1_run_batch")
+    assert q1.result_to_list(outs[0]) == ctx.run(comp, 5, rows - 10, 729950)
+    assert q1.result_to_list(outs[1]) == ctx.run(comp, 0, 77, 729950)
+    # a mix of layouts runs request by request, same answers
+    reqs = [q1.Request(comp.handle, 0, rows, 729999, 0), q1.Request(ref.handle, 3, 1000, 729999, 0)]
+    outs = (q1.Result * 2)()
+    ctx.check(q1.lib.q1_run_batch(ctx.handle, (q1.Request * 2)(*reqs), 2, None, outs), "q1_run_batch")
+    assert q1.result_to_list(outs[0]) == ctx.run(comp, 0, rows, 729999)
+    assert q1.result_to_list(outs[1]) == ctx.run(ref, 3, 1000, 729999)

@@ -1,0 +1,221 @@
+"""HIP path vs the oracle, bit-exact, through the C ABI (every layout x placement, edge
+sizes, row offsets, runtime cutoffs, launch shapes, out-of-domain values, 128-bit sums)."""
+import sys
+
+import numpy as np
+import pytest
+
+import pinning
+from tpchq01_gpu_amd import q1
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x51C00001
+LAYOUTS = [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED, q1.COMPRESSED_PUSHDOWN]
+PLACEMENTS = [q1.IN_REGISTERS, q1.IN_REGISTERS_PER_THREAD, q1.LOCAL_MEM, q1.PER_THREAD_SHARED_MEM, q1.GLOBAL]
+SIZES = [0, 1, 15, 16, 17, 31, 32, 33, 1000, 65_537, 1_000_003]
+CUTOFFS = [729999, 729906, 730088]   # 1998-09-02, 1998-06-01, 1998-11-30
+
+
+def as_dicts(groups):
+    return [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
+                 sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in groups]
+
+
+def note(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+_host_cache = {}
+
+
+def host_tables(rows):
+    if rows not in _host_cache:
+        _host_cache.clear()
+        _host_cache[rows] = {
+            q1.REFERENCE: q1.generate_host(q1.REFERENCE, SEED, rows),
+            q1.UNCOMPRESSED: q1.generate_host(q1.UNCOMPRESSED, SEED, rows),
+            q1.COMPRESSED: q1.generate_host(q1.COMPRESSED, SEED, rows),
+        }
+    return _host_cache[rows]
+
+
+def device_table(ctx, layout, rows, cutoff):
+    h = host_tables(rows)
+    if layout == q1.COMPRESSED_PUSHDOWN:
+        comp = ctx.upload(q1.COMPRESSED, h[q1.COMPRESSED], rows)
+        return ctx.build_filter(comp, cutoff)
+    return ctx.upload(layout, h[layout], rows)
+
+
+@pytest.mark.parametrize("rows", SIZES)
+def test_all_layouts_and_placements(ctx, oracle, rows):
+    ref = host_tables(rows)[q1.REFERENCE]
+    for cutoff in CUTOFFS:
+        want, _ = oracle.naive(ref, 0, rows, cutoff)
+        for layout in LAYOUTS:
+            t = device_table(ctx, layout, rows, cutoff)
+            for p in PLACEMENTS:
+                note(f"case rows={rows} cutoff={cutoff} layout={q1.LAYOUT_NAMES[layout]} placement={q1.PLACEMENT_NAMES[p]}")
+                got = ctx.run(t, 0, rows, cutoff, p)
+                assert as_dicts(got) == want
+            t.free()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_row_ranges(ctx, oracle, layout):
+    rows = 100_000
+    ref = host_tables(rows)[q1.REFERENCE]
+    cutoff = 729906
+    t = device_table(ctx, layout, rows, cutoff)
+    for begin, end in [(3, rows - 5), (17, rows), (0, rows - 1), (31, 33), (15, 16), (4097, 70001), (5, 5)]:
+        want, _ = oracle.naive(ref, begin, end, cutoff)
+        for p in PLACEMENTS:
+            note(f"case layout={q1.LAYOUT_NAMES[layout]} range=[{begin},{end}) placement={p}")
+            assert as_dicts(ctx.run(t, begin, end - begin, cutoff, p)) == want
+
+
+@pytest.mark.parametrize("shape", [(64, 16), (128, 256), (192, 48), (256, 4096), (256, 16384), (64, 1024)])
+def test_launch_shapes(ctx, oracle, shape):
+    rows = 300_001
+    ref = host_tables(rows)[q1.REFERENCE]
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    for layout in (q1.COMPRESSED, q1.UNCOMPRESSED, q1.REFERENCE):
+        t = device_table(ctx, layout, rows, 729999)
+        for p in PLACEMENTS:
+            note(f"case shape={shape} layout={layout} placement={p}")
+            got = ctx.run(t, 0, rows, 729999, p, q1.make_launch(*shape))
+            assert as_dicts(got) == want
+        t.free()
+
+
+def test_cutoff_extremes(ctx, oracle):
+    rows = 50_000
+    ref = host_tables(rows)[q1.REFERENCE]
+    t = device_table(ctx, q1.COMPRESSED, rows, 729999)
+    for cutoff in (0, 727563 - 2, 727563, 727564, 730089, 727563 + 65535, 727563 + 70000, 2**31 - 1):
+        want, _ = oracle.naive(ref, 0, rows, cutoff)
+        for p in PLACEMENTS:
+            assert as_dicts(ctx.run(t, 0, rows, cutoff, p)) == want, (cutoff, p)
+
+
+def test_device_generator_matches_host(ctx):
+    for layout in (q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED):
+        for rows in (1, 33, 1000, 100_017):
+            t = ctx.generate(layout, SEED + 9, rows)
+            got = t.download()
+            want = q1.generate_host(layout, SEED + 9, rows)
+            for k in want:
+                assert np.array_equal(got[k], want[k]), (layout, rows, k)
+            t.free()
+
+
+def test_device_filter_matches_oracle(ctx, oracle):
+    rows = 77_777
+    comp = host_tables(rows)[q1.COMPRESSED]
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    for cutoff in CUTOFFS + [727000, 740000]:
+        pd = ctx.build_filter(t, cutoff)
+        got = pd.download()["filter"]
+        assert np.array_equal(got, oracle.precompute_filter(comp["ship_date"], rows, cutoff))
+
+
+def test_uploaded_pushdown_columns(ctx, oracle):
+    rows = 65_536 + 77
+    ref = host_tables(rows)[q1.REFERENCE]
+    comp = host_tables(rows)[q1.COMPRESSED]
+    pdc = q1.pushdown_columns(comp, 729906, rows)
+    t = ctx.upload(q1.COMPRESSED_PUSHDOWN, pdc, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729906)
+    for p in PLACEMENTS:
+        # the pushdown table's bitmap holds the filter; the cutoff argument is not used
+        assert as_dicts(ctx.run(t, 0, rows, 0, p)) == want
+
+
+def test_out_of_domain_values_take_the_64bit_path(ctx, oracle):
+    rng = np.random.default_rng(5)
+    rows = pinning.random_rows(rng, 20_000, tpch=False)
+    cols = pinning.ref_columns(rows)
+    want, _ = oracle.naive(cols, 0, len(rows), 729999)
+    t = ctx.upload(q1.REFERENCE, cols, len(rows))
+    for p in PLACEMENTS:
+        assert as_dicts(ctx.run(t, 0, len(rows), 729999, p)) == want
+    assert ctx.last_timing().placement == q1.IN_REGISTERS_PER_THREAD
+
+
+def test_compressed_out_of_domain(ctx, oracle):
+    # discount above 100 makes (100 - disc) negative: the 64-bit kernel must take over
+    rows = 10_000
+    comp = {k: v.copy() for k, v in host_tables(rows)[q1.COMPRESSED].items()}
+    ref = {k: v.copy() for k, v in host_tables(rows)[q1.REFERENCE].items()}
+    comp["discount"][::7] = 200
+    ref["discount"][::7] = 200
+    comp["extended_price"][::11] = 4_000_000_000
+    ref["extended_price"][::11] = 4_000_000_000
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    for p in PLACEMENTS:
+        assert as_dicts(ctx.run(t, 0, rows, 729999, p)) == want
+
+
+def test_int128_sums_cross_2_64_general_path(ctx, oracle):
+    rows = [(729999, 1, 9 * 10**14, 0, 0, "N", "O")] * 4000 + [(729999, 1, 9 * 10**14, 200, 3, "R", "F")] * 4000
+    cols = pinning.ref_columns(rows)
+    want, _ = oracle.naive(cols, 0, len(rows), 729999)
+    assert want[1 << 1 | 1]["sum_charge"] > 2**64 and want[2 << 1]["sum_charge"] < -(2**64)
+    t = ctx.upload(q1.REFERENCE, cols, len(rows))
+    for p in PLACEMENTS:
+        assert as_dicts(ctx.run(t, 0, len(rows), 729999, p)) == want
+
+
+@pytest.mark.slow
+def test_int128_carry_fast_path(ctx):
+    """100M rows of the largest TPC-H-domain values in one group: sum_charge passes 2^64,
+    so the carry of the per-block 128-bit merge is exercised.  Closed form."""
+    n = 100_000_000
+    price, disc, tax, qty = (1 << 24) - 1, 0, 15, 63
+    cols = {
+        "ship_date": np.zeros(n, np.uint16), "quantity": np.full(n, qty, np.uint8),
+        "extended_price": np.full(n, price, np.uint32), "discount": np.full(n, disc, np.uint8),
+        "tax": np.full(n, tax, np.uint8),
+        "return_flag": np.full((n + 15) // 16, 0x55555555, np.uint32),     # every row 'N'
+        "line_status": np.full((n + 31) // 32, 0xFFFFFFFF, np.uint32),     # every row 'O'
+    }
+    t = ctx.upload(q1.COMPRESSED, cols, n)
+    charge = n * (100 - disc) * price * (100 + tax)
+    assert charge > 2**64
+    for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM, q1.IN_REGISTERS_PER_THREAD):
+        g = ctx.run(t, 0, n, 729999, p)[1 << 1 | 1]
+        assert (g.count, g.sum_qty, g.sum_base_price, g.sum_disc) == (n, n * qty * 100, n * price, 0)
+        assert g.sum_disc_price == n * (100 - disc) * price
+        assert g.sum_charge == charge
+    t.free()
+
+
+def test_split_matches_oracle(ctx, oracle):
+    rows = 1_000_003
+    h = host_tables(rows)
+    want, _ = oracle.naive(h[q1.REFERENCE], 0, rows, 729999)
+    for layout in (q1.COMPRESSED, q1.REFERENCE):
+        t = ctx.upload(layout, h[layout], rows)
+        for frac in (0.0, 0.25, 0.5, 1.0):
+            got, cms, gms = ctx.run_split(t, h[layout], rows, frac, 729999, q1.IN_REGISTERS,
+                                          align_rows=1 << 10, cpu_threads=4)
+            assert as_dicts(got) == want, (layout, frac)
+        t.free()
+
+
+def test_streaming_matches_oracle(ctx, oracle):
+    rows = 3_000_017
+    h = host_tables(rows)
+    want, _ = oracle.naive(h[q1.REFERENCE], 0, rows, 729906)
+    for layout in (q1.COMPRESSED, q1.UNCOMPRESSED):
+        for streams, chunk in ((1, 1 << 20), (4, 1 << 18), (3, 96_000)):
+            for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM, q1.GLOBAL):
+                got = ctx.run_streaming(layout, h[layout], rows, 729906, p,
+                                        q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN), chunk, streams)
+                assert as_dicts(got) == want, (layout, streams, chunk, p)
+    # without the domain flag the 64-bit kernels run, same answer
+    got = ctx.run_streaming(q1.COMPRESSED, h[q1.COMPRESSED], rows, 729906, q1.IN_REGISTERS, None, 1 << 20, 2)
+    assert as_dicts(got) == want

@@ -1,0 +1,148 @@
+"""Host-side product code against the oracle, on the CPU (no GPU needed): the seeded
+generator, the column compression (src/main.cu:773-803), the pushdown producer
+(src/main.cu:362-389), the CPU coprocessor aggregation and the date numbering."""
+import datetime
+
+import numpy as np
+import pytest
+
+from tpchq01_gpu_amd import q1
+
+SEED = 0x51C00000
+
+
+def as_ref_layout(cols: dict) -> dict:
+    """UNCOMPRESSED columns widened to the reference layout (same values)."""
+    return {
+        "ship_date": cols["ship_date"].astype(np.int32),
+        "quantity": cols["quantity"].astype(np.int64),
+        "extended_price": cols["extended_price"].astype(np.int64),
+        "discount": cols["discount"].astype(np.int64),
+        "tax": cols["tax"].astype(np.int64),
+        "return_flag": cols["return_flag"],
+        "line_status": cols["line_status"],
+    }
+
+
+def as_dicts(groups):
+    return [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
+                 sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in groups]
+
+
+def test_day_number_matches_oracle(oracle):
+    d = datetime.date(1900, 1, 1)
+    while d.year < 2100:
+        assert q1.day_number(d.year, d.month, d.day) == oracle.todate(d.day, d.month, d.year)
+        d += datetime.timedelta(days=29)
+    assert q1.day_number(1998, 9, 2) == 729999
+
+
+@pytest.mark.parametrize("rows", [0, 1, 31, 32, 33, 1000, 100_003])
+def test_generator_layouts_agree(oracle, rows):
+    ref = q1.generate_host(q1.REFERENCE, SEED, rows)
+    unc = q1.generate_host(q1.UNCOMPRESSED, SEED, rows)
+    comp = q1.generate_host(q1.COMPRESSED, SEED, rows)
+    assert np.array_equal(as_ref_layout(unc)["quantity"], ref["quantity"])
+    for k in ("ship_date", "extended_price", "return_flag", "line_status"):
+        assert np.array_equal(unc[k].astype(np.int64), ref[k].astype(np.int64)), k
+    for k in ("discount", "tax"):
+        assert np.array_equal(unc[k].astype(np.int64), ref[k]), k
+    if rows:
+        want = oracle.compress(ref, rows)
+        for k in want:
+            assert np.array_equal(comp[k], want[k]), k
+
+
+def test_generator_row_ranges_regenerate():
+    full = q1.generate_host(q1.COMPRESSED, SEED, 4096)
+    part = q1.generate_host(q1.COMPRESSED, SEED, 1024, row_begin=2048)
+    for k in ("ship_date", "quantity", "extended_price", "discount", "tax"):
+        assert np.array_equal(full[k][2048:3072], part[k]), k
+    assert np.array_equal(full["return_flag"][128:192], part["return_flag"])
+    assert np.array_equal(full["line_status"][64:96], part["line_status"])
+
+
+def test_generator_distribution():
+    n = 400_000
+    ref = q1.generate_host(q1.REFERENCE, SEED + 1, n)
+    ship = ref["ship_date"]
+    assert ship.min() >= 727564 and ship.max() <= 730089
+    assert set(np.unique(ref["discount"])) <= set(range(11))
+    assert set(np.unique(ref["tax"])) <= set(range(9))
+    q = ref["quantity"] // 100
+    assert q.min() >= 1 and q.max() <= 50
+    part = ref["extended_price"] // q
+    assert np.all(ref["extended_price"] % q == 0) and part.min() >= 90000 and part.max() <= 209999
+    sel = (ship <= 729999).mean()
+    assert abs(sel - 0.9644) < 0.003          # uniform dates with a 90-day tail cut
+    rf = ref["return_flag"].view("S1")
+    ls = ref["line_status"].view("S1")
+    m = ship <= 729999
+    share = lambda a, b: ((rf == a) & (ls == b) & m).mean()
+    assert abs(share(b"A", b"F") - 0.2469) < 0.004
+    assert abs(share(b"R", b"F") - 0.2469) < 0.004
+    assert abs(share(b"N", b"F") - 0.0061) < 0.001
+    assert abs(share(b"N", b"O") - 0.4644) < 0.004
+    assert share(b"A", b"O") == 0 and share(b"R", b"O") == 0   # receipt > ship
+
+
+@pytest.mark.parametrize("rows", [1, 17, 64, 1000, 65_537])
+def test_compress_matches_oracle(oracle, rows):
+    ref = q1.generate_host(q1.REFERENCE, SEED + 2, rows)
+    got = q1.compress_columns(ref, rows, threads=3)
+    want = oracle.compress(ref, rows)
+    for k in want:
+        assert np.array_equal(got[k], want[k]), k
+
+
+@pytest.mark.parametrize("rows", [1, 31, 32, 33, 1000, 99_999])
+@pytest.mark.parametrize("cutoff", [729999, 729906, 730088, 727000])
+def test_cpu_filter_matches_oracle(oracle, rows, cutoff):
+    comp = q1.generate_host(q1.COMPRESSED, SEED + 3, rows)
+    got = q1.cpu_build_filter(comp["ship_date"], cutoff, threads=4)
+    assert np.array_equal(got, oracle.precompute_filter(comp["ship_date"], rows, cutoff))
+
+
+@pytest.mark.parametrize("layout", [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED, q1.COMPRESSED_PUSHDOWN])
+@pytest.mark.parametrize("rng", [(0, 0), (0, 1), (5, 1000), (0, 65_536), (17, 200_000)])
+def test_cpu_coprocessor_matches_oracle(oracle, layout, rng):
+    begin, count = rng
+    n = begin + count
+    ref = q1.generate_host(q1.REFERENCE, SEED + 4, n)
+    want, unexpected = oracle.naive(ref, begin, n, 729906)
+    assert unexpected == 0
+    if layout == q1.REFERENCE:
+        cols = ref
+    elif layout == q1.UNCOMPRESSED:
+        cols = q1.generate_host(q1.UNCOMPRESSED, SEED + 4, n)
+    else:
+        cols = q1.generate_host(q1.COMPRESSED, SEED + 4, n)
+        if layout == q1.COMPRESSED_PUSHDOWN:
+            cols = q1.pushdown_columns(cols, 729906, n)
+    got = q1.cpu_run(layout, cols, begin, count, 729906, threads=4)
+    assert as_dicts(got) == want
+
+
+def test_cpu_coprocessor_out_of_domain(oracle):
+    """Values far outside TPC-H (negative, huge): exact int64/int128 semantics."""
+    import pinning
+    rng = np.random.default_rng(1)
+    rows = pinning.random_rows(rng, 3000, tpch=False)
+    cols = pinning.ref_columns(rows)
+    want, _ = oracle.naive(cols, 0, 3000, 729999)
+    assert as_dicts(q1.cpu_run(q1.REFERENCE, cols, 0, 3000, 729999, threads=3)) == want
+
+
+def test_result_merge_is_exact():
+    a, b, out = q1.Result(), q1.Result(), q1.Result()
+    a.g[0].sum_charge_lo = 2**64 - 5
+    a.g[0].sum_charge_hi = 1
+    b.g[0].sum_charge_lo = 10
+    b.g[0].sum_charge_hi = -3
+    a.g[1].sum_disc_price_lo = 1
+    b.g[1].sum_disc_price_lo = 2**64 - 1
+    b.g[1].sum_disc_price_hi = -1
+    q1.lib.q1_result_merge(a, b, out)
+    r = q1.result_to_list(out)
+    assert r[0].sum_charge == (2**64 + 2**64 - 5) + (-3 * 2**64 + 10)
+    assert r[1].sum_disc_price == 1 + (-(2**64) + 2**64 - 1)

@@ -1,0 +1,169 @@
+// q1_run_batch: many filter-aggregate requests (uneven row counts, their own cutoffs) in
+// one persistent launch, captured once into a hipGraph and replayed:
+//   H2D copy of the request descriptors (pinned staging) -> zero the per-request tables
+//   (a fill kernel, never a memset node) -> the batched kernel -> D2H copy of the tables.
+// The graph is rebuilt only when the capacity bucket, layout, placement or grid changes;
+// each call only rewrites the pinned descriptor buffer before the replay.
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "q1_internal.h"
+
+struct q1_batch_state {
+  q1_layout layout = Q1_LAYOUT_COMPRESSED;
+  q1_placement placement = Q1_PLACE_IN_REGISTERS;
+  uint32_t capacity = 0, blocks = 0, threads = 0;
+  q1::BatchHeader* h_buf = nullptr;  // pinned: header + capacity descriptors
+  q1::BatchHeader* d_buf = nullptr;
+  q1_device_aggregates* h_out = nullptr;  // pinned
+  q1_device_aggregates* d_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace q1 {
+
+void batch_destroy(q1_batch_state* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->h_buf) (void)hipHostFree(b->h_buf);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->d_buf) (void)hipFree(b->d_buf);
+  if (b->d_out) (void)hipFree(b->d_out);
+  delete b;
+}
+
+static uint64_t buf_bytes(uint32_t cap) { return sizeof(BatchHeader) + (uint64_t)cap * sizeof(BatchDesc); }
+
+static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placement, uint32_t cap,
+                             uint32_t threads, uint32_t blocks) {
+  q1_batch_state* b = ctx->batch;
+  if (b && b->layout == layout && b->placement == placement && b->capacity == cap && b->threads == threads &&
+      b->blocks == blocks)
+    return Q1_OK;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, Q1_EDEVICE, "stream sync failed");
+  batch_destroy(b);
+  ctx->batch = nullptr;
+  b = new q1_batch_state();
+  b->layout = layout;
+  b->placement = placement;
+  b->capacity = cap;
+  b->threads = threads;
+  b->blocks = blocks;
+  bool ok = hipHostMalloc(&b->h_buf, buf_bytes(cap), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&b->h_out, cap * sizeof(q1_device_aggregates), hipHostMallocDefault) == hipSuccess &&
+            hipMalloc(&b->d_buf, buf_bytes(cap)) == hipSuccess &&
+            hipMalloc(&b->d_out, cap * sizeof(q1_device_aggregates)) == hipSuccess;
+  if (!ok) {
+    batch_destroy(b);
+    return fail(ctx, Q1_ENOMEM, "batch buffers could not be allocated");
+  }
+  std::memset(b->h_buf, 0, buf_bytes(cap));
+  // Capture the whole step once.
+  hipStream_t s = ctx->stream;
+  q1_status st = Q1_OK;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) st = Q1_EDEVICE;
+  if (st == Q1_OK && hipMemcpyAsync(b->d_buf, b->h_buf, buf_bytes(cap), hipMemcpyHostToDevice, s) != hipSuccess)
+    st = Q1_EDEVICE;
+  if (st == Q1_OK) st = launch_zero(b->d_out, cap, s);
+  if (st == Q1_OK) st = launch_batch(layout, placement, b->d_buf, b->d_out, blocks, threads, s);
+  if (st == Q1_OK &&
+      hipMemcpyAsync(b->h_out, b->d_out, cap * sizeof(q1_device_aggregates), hipMemcpyDeviceToHost, s) != hipSuccess)
+    st = Q1_EDEVICE;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (st != Q1_OK || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    batch_destroy(b);
+    return fail(ctx, Q1_EDEVICE, "capturing the batch graph failed");
+  }
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    batch_destroy(b);
+    return fail(ctx, Q1_EDEVICE, "instantiating the batch graph failed");
+  }
+  ctx->batch = b;
+  return Q1_OK;
+}
+
+}  // namespace q1
+
+using namespace q1;
+
+extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t n, const q1_launch* launch,
+                                  q1_result* outs) {
+  if (!ctx || (!reqs && n) || (!outs && n)) return fail(ctx, Q1_EINVAL, "bad argument");
+  if (n == 0) return Q1_OK;
+  if (n > (1u << 16)) return fail(ctx, Q1_EINVAL, "at most 65536 requests per batch");
+  for (uint32_t i = 0; i < n; i++) {
+    const q1_table* t = reqs[i].table;
+    if (!t) return fail(ctx, Q1_EINVAL, "request without a table");
+    if (reqs[i].row_begin > t->rows || reqs[i].row_count > t->rows - reqs[i].row_begin)
+      return fail(ctx, Q1_EINVAL, "request row range outside its table");
+  }
+  try {
+    // One persistent launch needs one layout and the TPC-H domain; any other mix runs
+    // request by request (same results, no batching).
+    const q1_layout layout = reqs[0].table->layout;
+    bool batchable = true;
+    for (uint32_t i = 0; i < n; i++)
+      batchable = batchable && reqs[i].table->layout == layout && reqs[i].table->fast;
+    if (!batchable) {
+      for (uint32_t i = 0; i < n; i++) {
+        const q1_status s = q1_run(ctx, reqs[i].table, reqs[i].row_begin, reqs[i].row_count, reqs[i].cutoff_day,
+                                   Q1_PLACE_AUTO, nullptr, &outs[i]);
+        if (s != Q1_OK) return s;
+      }
+      return Q1_OK;
+    }
+    std::lock_guard<std::mutex> g(ctx->lock);
+    q1_placement p = ctx->best[layout];
+    if (p != Q1_PLACE_PER_THREAD_SHARED_MEM) p = Q1_PLACE_IN_REGISTERS;
+    const uint32_t threads = launch && launch->threads_per_block ? launch->threads_per_block : 256;
+    if (threads % 64 || threads > 256) return fail(ctx, Q1_EINVAL, "threads_per_block must be 64..256, multiple of 64");
+    const uint32_t blocks = launch && launch->num_blocks ? launch->num_blocks
+                                                         : (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 4;
+    uint32_t cap = 16;
+    while (cap < n) cap <<= 1;
+    q1_status s = build_state(ctx, layout, p, cap, threads, blocks);
+    if (s != Q1_OK) return s;
+    q1_batch_state* b = ctx->batch;
+    const uint64_t trows = layout_tile_rows(layout);
+    BatchDesc* d = reinterpret_cast<BatchDesc*>(b->h_buf + 1);
+    uint64_t tiles = 0, rows = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      const q1_table* t = reqs[i].table;
+      d[i].cols = t->dev;
+      d[i].row_begin = reqs[i].row_begin;
+      d[i].row_end = reqs[i].row_begin + reqs[i].row_count;
+      d[i].tile_begin = tiles;
+      d[i].cutoff_day = t->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t->filter_cutoff : reqs[i].cutoff_day;
+      d[i].pad = 0;
+      if (reqs[i].row_count) tiles += (d[i].row_end + trows - 1) / trows - d[i].row_begin / trows;
+      rows += reqs[i].row_count;
+    }
+    b->h_buf->n = n;
+    b->h_buf->total_tiles = tiles;
+    const auto w0 = std::chrono::steady_clock::now();
+    if (hipEventRecord(ctx->ev_begin, ctx->stream) != hipSuccess ||
+        hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      return fail(ctx, Q1_EDEVICE, "batch graph launch failed");
+    for (uint32_t i = 0; i < n; i++) q1_aggregates_to_result(&b->h_out[i], &outs[i]);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);
+    ctx->last = q1_timing{};
+    ctx->last.kernel_ms = ms;
+    ctx->last.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    ctx->last.rows = rows;
+    ctx->last.bytes_read = algorithmic_bytes(layout, rows);
+    ctx->last.num_blocks = blocks;
+    ctx->last.threads_per_block = threads;
+    ctx->last.placement = (uint32_t)p;
+    return Q1_OK;
+  } catch (const std::exception& e) {
+    return fail(ctx, Q1_ENOMEM, e.what());
+  }
+}

@@ -1,0 +1,54 @@
+// Internal host-side state shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/q1_hip.h"
+#include "q1_kernels.h"
+
+struct q1_table {
+  q1_ctx* ctx = nullptr;
+  q1_layout layout = Q1_LAYOUT_COMPRESSED;
+  uint64_t rows = 0;
+  q1_columns dev{};
+  void* owned[8] = {};           // allocations this table frees
+  q1::Domain domain{};
+  bool fast = false;             // inside the TPC-H value domain
+  uint64_t device_bytes = 0;
+  int32_t filter_cutoff = 0;     // pushdown tables: the cutoff their bitmap encodes
+};
+
+struct q1_batch_state;  // q1_batch.cpp
+
+struct q1_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_h2d_begin = nullptr, ev_h2d_end = nullptr;
+  q1_device_aggregates* d_aggr = nullptr;  // one table for q1_run
+  q1_device_aggregates* h_aggr = nullptr;  // pinned host copy
+  int64_t* d_stats = nullptr;              // 8 words for domain statistics
+  void* workspace = nullptr;
+  uint64_t workspace_bytes = 0;
+  q1_timing last{};
+  std::string error;
+  int32_t compute_units = 0;
+  q1_launch tuning[4][5]{};                // [layout][placement]
+  q1_placement best[4] = {Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS,
+                          Q1_PLACE_IN_REGISTERS};
+  q1_batch_state* batch = nullptr;
+  std::mutex lock;                         // serialises calls that share the context stream
+};
+
+namespace q1 {
+q1_status fail(q1_ctx* ctx, q1_status s, const std::string& why);
+q1_status hip_fail(q1_ctx* ctx, hipError_t e, const char* what);
+q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p);
+void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, const q1_launch* in, q1_launch* out);
+uint64_t algorithmic_bytes(q1_layout layout, uint64_t rows);
+q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
+void batch_destroy(q1_batch_state* b);
+void stream_state_release(q1_ctx* ctx);
+}  // namespace q1

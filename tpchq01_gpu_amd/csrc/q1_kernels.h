@@ -1,0 +1,76 @@
+// Internal interface between the C-ABI layer (q1_api.cpp) and the gfx950 kernels
+// (q1_kernels.hip).  Host-only declarations; no HIP types leak into the public header.
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/q1_hip.h"
+
+namespace q1 {
+
+// Value domain of a table, measured on the device when the table is created.  Inside the
+// TPC-H domain the narrow 32-bit kernels are exact; outside it the 64-bit kernels run.
+struct Domain {
+  int64_t min_disc, max_disc, min_tax, max_tax, min_qty, max_qty, min_price, max_price;
+};
+
+bool domain_is_tpch(q1_layout layout, const Domain& d);
+
+struct LaunchShape {
+  uint32_t blocks, threads, tiles_per_thread;
+};
+
+// Kernel-level request: accumulate rows [row_begin, row_end) of `cols` into `out`.
+struct ScanArgs {
+  q1_columns cols;          // device pointers
+  uint64_t row_begin, row_end;
+  int32_t cutoff_day;       // MonetDB day number (ignored for pushdown)
+  q1_device_aggregates* out;
+};
+
+// Validate and complete a launch shape for `rows` rows (0 fields filled from defaults).
+q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* launch,
+                     uint64_t row_begin, uint64_t row_end, LaunchShape* shape, const char** why);
+
+// Enqueue the scan kernel.  `fast` selects the narrow (TPC-H domain) kernels.
+q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
+                      const LaunchShape& shape, void* workspace, void* stream);
+uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape);
+
+// Zero a device aggregate table (a fill kernel, safe inside graph capture).
+q1_status launch_zero(q1_device_aggregates* table, uint32_t count, void* stream);
+
+// Device-side generator: rows [0, rows) of `layout` into the device columns.
+q1_status launch_generate(q1_layout layout, uint64_t seed, uint64_t rows, const q1_columns& cols,
+                          void* stream);
+
+// Device-side domain statistics of a table into `out_dev` (8 int64 slots: min/max pairs).
+q1_status launch_domain(q1_layout layout, const q1_columns& cols, uint64_t rows, int64_t* out_dev,
+                        void* stream);
+
+// Device-side pushdown producer: filter bitmap of a compressed ship_date column.
+q1_status launch_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutoff_day,
+                              uint32_t* filter, void* stream);
+
+// Batched scan: one persistent launch over many requests of one layout, TPC-H domain.
+// The device buffer starts with a BatchHeader followed by `n` BatchDesc records; the grid
+// is fixed, each block takes an equal share of the concatenated tile space of all
+// requests (requests in order, tiles of a request in row order) and merges its partial
+// table into outs[r] whenever its share leaves request r.
+struct BatchDesc {
+  q1_columns cols;
+  uint64_t row_begin, row_end;
+  uint64_t tile_begin;        // first tile of this request in the concatenated tile space
+  int32_t cutoff_day;
+  uint32_t pad;
+};
+struct BatchHeader {
+  uint64_t n;                 // requests
+  uint64_t total_tiles;       // tiles of all requests
+  uint64_t pad[6];
+};
+uint32_t layout_tile_rows(q1_layout layout);
+q1_status launch_batch(q1_layout layout, q1_placement placement, const BatchHeader* dev_buffer,
+                       q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream);
+
+}  // namespace q1

@@ -1,0 +1,1201 @@
+// gfx950 kernels of the TPC-H Q1 scan -> filter -> group-aggregate path.
+//
+// Work decomposition (all layouts, all placements):
+//   * rows are cut into lane tiles of TILE consecutive rows (16 for the compressed layouts,
+//     so that one 16-byte load carries 16 uint8 values, one uint32 word 16 packed return
+//     flags and one uint16 half-word 16 packed line statuses / filter bits);
+//   * a block owns tiles_per_thread * blockDim consecutive tiles; in iteration k thread i
+//     takes tile base + k*blockDim + i, so each load instruction of a wavefront covers one
+//     contiguous stretch of every column (coalesced 16-byte loads);
+//   * only the first and the last tile of a range can be partial: those take a guarded
+//     per-row path, every other tile reads whole vectors;
+//   * each placement keeps its partial aggregation table in its own memory space
+//     (registers, scratch, LDS or global memory); at the end of the block the per-lane
+//     partials are reduced across the wavefront with ds_bpermute butterflies, across the
+//     block through LDS in 128-bit arithmetic, and merged into the destination table with
+//     one set of 64-bit atomics per block (128-bit sums as carry-propagating word pairs).
+//
+// Arithmetic (cpu/kernel_naive.hpp:14-25): disc_1 = 100 - disc, disc_price = disc_1*price,
+// charge = disc_price*(100 + tax), all exact.  Inside the TPC-H value domain (discount and
+// tax <= 15, quantity <= 63 units, price < 2^24; measured per table) the "fast" kernels use
+// 24-bit multiplies and narrow per-tile accumulators:
+//   small    = 1 | disc << 5 | qty << 13       (count, discount, quantity in one dword)
+//   dprod    = disc * price                     (then sum_disc_price = 100*sum_price - sum_dprod)
+//   charge   = (100-disc)*(100+tax) * price     (v_mul_u32_u24 + v_mul_hi_u32_u24)
+// Outside the domain the "general" kernel computes in wrapping int64 with 128-bit
+// signed sums, which is the reference CPU kernels' arithmetic for any input.
+#include <hip/hip_runtime.h>
+
+#include "q1_common.h"
+#include "q1_kernels.h"
+
+namespace q1 {
+namespace {
+
+constexpr int kMaxBlock = 256;
+constexpr uint32_t kNoGroup = 7;
+
+// ------------------------------------------------------------------------------------
+// Small helpers
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t comp(const uint4& v, int i) {
+  return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ void comp_or(uint4& v, int i, uint32_t bits) {
+  if (i == 0) v.x |= bits; else if (i == 1) v.y |= bits; else if (i == 2) v.z |= bits; else v.w |= bits;
+}
+__device__ __forceinline__ void comp_set(uint4& v, int i, uint32_t bits) {
+  if (i == 0) v.x = bits; else if (i == 1) v.y = bits; else if (i == 2) v.z = bits; else v.w = bits;
+}
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// Rows of the tile that lie inside [lo, hi): bit j set for row tile*TILE + j.
+template <int TILE>
+__device__ __forceinline__ uint32_t valid_mask(uint64_t tile, uint64_t lo, uint64_t hi) {
+  const uint64_t r0 = tile * TILE;
+  uint32_t m = 0;
+#pragma unroll
+  for (int j = 0; j < TILE; j++) m |= (uint32_t)(r0 + j >= lo && r0 + j < hi) << j;
+  return m;
+}
+
+// One row of the fast path: values already narrowed.  gid is the raw 3-bit slot code
+// (6 and 7 only arise from the unused return-flag code 3 and never count).
+struct FastRow {
+  bool pass;     // inside the row range and ship_date <= cutoff
+  uint32_t gid;
+  uint32_t qty, disc, tax, price;
+  __device__ __forceinline__ bool counts() const { return pass && gid < kNumGroups; }
+};
+
+// One row of the general path.
+struct WideRow {
+  uint32_t gid;
+  int64_t qty, price, disc, tax;
+};
+
+// ------------------------------------------------------------------------------------
+// Tile loaders, one per layout.  load() reads a whole tile with vector loads; load_edge()
+// reads only the rows inside [lo, hi) (guarded, per row) into the same vector form.
+// Accessors extract row j (j is a compile-time constant after unrolling).
+// ------------------------------------------------------------------------------------
+
+// Q1_LAYOUT_COMPRESSED / Q1_LAYOUT_COMPRESSED_PUSHDOWN: 16 rows per tile.
+template <bool PUSHDOWN>
+struct CompressedTile {
+  static constexpr int TILE = 16;
+  uint4 sd[2];      // 16 x uint16 ship dates (unused for pushdown)
+  uint4 q, d, x;    // 16 x uint8 quantity, discount, tax
+  uint4 p[4];       // 16 x uint32 extended price
+  uint32_t rf;      // 16 x 2-bit return flag codes
+  uint32_t ls;      // 16 x 1-bit line status codes (low half-word)
+  uint32_t flt;     // 16 x filter bits (pushdown)
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    if (!PUSHDOWN) {
+      const uint4* s = reinterpret_cast<const uint4*>(c.ship_date);
+      sd[0] = s[2 * t];
+      sd[1] = s[2 * t + 1];
+    } else {
+      flt = reinterpret_cast<const uint16_t*>(c.filter)[t];
+    }
+    q = reinterpret_cast<const uint4*>(c.quantity)[t];
+    d = reinterpret_cast<const uint4*>(c.discount)[t];
+    x = reinterpret_cast<const uint4*>(c.tax)[t];
+    const uint4* pp = reinterpret_cast<const uint4*>(c.extended_price);
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = pp[4 * t + k];
+    rf = reinterpret_cast<const uint32_t*>(c.return_flag)[t];
+    ls = reinterpret_cast<const uint16_t*>(c.line_status)[t];
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    sd[0] = sd[1] = q = d = x = p[0] = p[1] = p[2] = p[3] = make_uint4(0, 0, 0, 0);
+    rf = ls = flt = 0;
+    // The packed words of a tile hold rows of the tile only, so reading them for a tile
+    // that has at least one valid row stays inside the column.
+    if (valid) {
+      rf = reinterpret_cast<const uint32_t*>(c.return_flag)[t];
+      ls = reinterpret_cast<const uint16_t*>(c.line_status)[t];
+      if (PUSHDOWN) flt = reinterpret_cast<const uint16_t*>(c.filter)[t];
+    }
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        if (!PUSHDOWN) {
+          const uint32_t s = reinterpret_cast<const uint16_t*>(c.ship_date)[r];
+          comp_or(sd[j >> 3], (j >> 1) & 3, s << (16 * (j & 1)));
+        }
+        comp_or(q, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.quantity)[r] << (8 * (j & 3)));
+        comp_or(d, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.discount)[r] << (8 * (j & 3)));
+        comp_or(x, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.tax)[r] << (8 * (j & 3)));
+        comp_set(p[j >> 2], j & 3, reinterpret_cast<const uint32_t*>(c.extended_price)[r]);
+      }
+    }
+  }
+  // `cut` is the cutoff relative to the frame of reference (may be < 0 or > 65535).
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const {
+    if (PUSHDOWN) return (flt >> j) & 1;
+    const uint32_t s = (comp(sd[j >> 3], (j >> 1) & 3) >> (16 * (j & 1))) & 0xFFFFu;
+    return (int32_t)s <= cut;
+  }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return ((rf >> (2 * j)) & 3u) << 1 | ((ls >> j) & 1u);
+  }
+  __device__ __forceinline__ uint32_t byte(const uint4& v, int j) const {
+    return (comp(v, j >> 2) >> (8 * (j & 3))) & 0xFFu;
+  }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = byte(q, j);
+    r.disc = byte(d, j);
+    r.tax = byte(x, j);
+    r.price = comp(p[j >> 2], j & 3);
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = byte(q, j);
+    r.disc = byte(d, j);
+    r.tax = byte(x, j);
+    r.price = comp(p[j >> 2], j & 3);
+    return r;
+  }
+};
+
+// Q1_LAYOUT_UNCOMPRESSED: int32 ship, int32 qty, int64 price, int8 disc/tax, char rf/ls.
+struct UncompressedTile {
+  static constexpr int TILE = 8;
+  uint4 s[2], q[2], p[4];
+  uint2 d, x, rf, ls;
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    const uint4* ss = reinterpret_cast<const uint4*>(c.ship_date);
+    const uint4* qq = reinterpret_cast<const uint4*>(c.quantity);
+    const uint4* pp = reinterpret_cast<const uint4*>(c.extended_price);
+    s[0] = ss[2 * t]; s[1] = ss[2 * t + 1];
+    q[0] = qq[2 * t]; q[1] = qq[2 * t + 1];
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = pp[4 * t + k];
+    d = reinterpret_cast<const uint2*>(c.discount)[t];
+    x = reinterpret_cast<const uint2*>(c.tax)[t];
+    rf = reinterpret_cast<const uint2*>(c.return_flag)[t];
+    ls = reinterpret_cast<const uint2*>(c.line_status)[t];
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    s[0] = s[1] = q[0] = q[1] = p[0] = p[1] = p[2] = p[3] = make_uint4(0, 0, 0, 0);
+    d = x = rf = ls = make_uint2(0, 0);
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        comp_set(s[j >> 2], j & 3, (uint32_t)reinterpret_cast<const int32_t*>(c.ship_date)[r]);
+        comp_set(q[j >> 2], j & 3, (uint32_t)reinterpret_cast<const int32_t*>(c.quantity)[r]);
+        const uint64_t pr = (uint64_t)reinterpret_cast<const int64_t*>(c.extended_price)[r];
+        comp_set(p[j >> 1], 2 * (j & 1), (uint32_t)pr);
+        comp_set(p[j >> 1], 2 * (j & 1) + 1, (uint32_t)(pr >> 32));
+        const int sh = 8 * (j & 3);
+        uint32_t* dd = (j < 4) ? &d.x : &d.y;
+        uint32_t* xx = (j < 4) ? &x.x : &x.y;
+        uint32_t* rr = (j < 4) ? &rf.x : &rf.y;
+        uint32_t* ll = (j < 4) ? &ls.x : &ls.y;
+        *dd |= (uint32_t)reinterpret_cast<const uint8_t*>(c.discount)[r] << sh;
+        *xx |= (uint32_t)reinterpret_cast<const uint8_t*>(c.tax)[r] << sh;
+        *rr |= (uint32_t)reinterpret_cast<const uint8_t*>(c.return_flag)[r] << sh;
+        *ll |= (uint32_t)reinterpret_cast<const uint8_t*>(c.line_status)[r] << sh;
+      }
+    }
+  }
+  __device__ __forceinline__ static uint32_t byte8(const uint2& v, int j) {
+    return ((j < 4 ? v.x : v.y) >> (8 * (j & 3))) & 0xFFu;
+  }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return encode_return_flag(byte8(rf, j)) << 1 | encode_line_status(byte8(ls, j));
+  }
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const {
+    return (int32_t)comp(s[j >> 2], j & 3) <= cut;
+  }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = comp(q[j >> 2], j & 3);
+    r.disc = byte8(d, j);
+    r.tax = byte8(x, j);
+    r.price = comp(p[j >> 1], 2 * (j & 1));
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = (int32_t)comp(q[j >> 2], j & 3);
+    r.disc = (int8_t)byte8(d, j);
+    r.tax = (int8_t)byte8(x, j);
+    r.price = (int64_t)((uint64_t)comp(p[j >> 1], 2 * (j & 1) + 1) << 32 | comp(p[j >> 1], 2 * (j & 1)));
+    return r;
+  }
+};
+
+// Q1_LAYOUT_REFERENCE: int32 ship, int64 qty/price/disc/tax, char rf/ls.
+struct ReferenceTile {
+  static constexpr int TILE = 4;
+  uint4 s, q[2], p[2], d[2], x[2];
+  uint32_t rf, ls;
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    s = reinterpret_cast<const uint4*>(c.ship_date)[t];
+    const uint4* qq = reinterpret_cast<const uint4*>(c.quantity);
+    const uint4* pp = reinterpret_cast<const uint4*>(c.extended_price);
+    const uint4* dd = reinterpret_cast<const uint4*>(c.discount);
+    const uint4* xx = reinterpret_cast<const uint4*>(c.tax);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      q[k] = qq[2 * t + k];
+      p[k] = pp[2 * t + k];
+      d[k] = dd[2 * t + k];
+      x[k] = xx[2 * t + k];
+    }
+    rf = reinterpret_cast<const uint32_t*>(c.return_flag)[t];
+    ls = reinterpret_cast<const uint32_t*>(c.line_status)[t];
+  }
+  __device__ __forceinline__ static void put64(uint4* v, int j, uint64_t val) {
+    comp_set(v[j >> 1], 2 * (j & 1), (uint32_t)val);
+    comp_set(v[j >> 1], 2 * (j & 1) + 1, (uint32_t)(val >> 32));
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    s = q[0] = q[1] = p[0] = p[1] = d[0] = d[1] = x[0] = x[1] = make_uint4(0, 0, 0, 0);
+    rf = ls = 0;
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        comp_set(s, j, (uint32_t)reinterpret_cast<const int32_t*>(c.ship_date)[r]);
+        put64(q, j, (uint64_t)reinterpret_cast<const int64_t*>(c.quantity)[r]);
+        put64(p, j, (uint64_t)reinterpret_cast<const int64_t*>(c.extended_price)[r]);
+        put64(d, j, (uint64_t)reinterpret_cast<const int64_t*>(c.discount)[r]);
+        put64(x, j, (uint64_t)reinterpret_cast<const int64_t*>(c.tax)[r]);
+        rf |= (uint32_t)reinterpret_cast<const uint8_t*>(c.return_flag)[r] << (8 * j);
+        ls |= (uint32_t)reinterpret_cast<const uint8_t*>(c.line_status)[r] << (8 * j);
+      }
+    }
+  }
+  __device__ __forceinline__ static uint32_t lo32(const uint4* v, int j) {
+    return comp(v[j >> 1], 2 * (j & 1));
+  }
+  __device__ __forceinline__ static int64_t get64(const uint4* v, int j) {
+    return (int64_t)((uint64_t)comp(v[j >> 1], 2 * (j & 1) + 1) << 32 | comp(v[j >> 1], 2 * (j & 1)));
+  }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return encode_return_flag((rf >> (8 * j)) & 0xFFu) << 1 | encode_line_status((ls >> (8 * j)) & 0xFFu);
+  }
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const { return (int32_t)comp(s, j) <= cut; }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = lo32(q, j);
+    r.disc = lo32(d, j);
+    r.tax = lo32(x, j);
+    r.price = lo32(p, j);
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = get64(q, j);
+    r.disc = get64(d, j);
+    r.tax = get64(x, j);
+    r.price = get64(p, j);
+    return r;
+  }
+};
+
+template <int L> struct TileOf;
+template <> struct TileOf<Q1_LAYOUT_REFERENCE> { using T = ReferenceTile; };
+template <> struct TileOf<Q1_LAYOUT_UNCOMPRESSED> { using T = UncompressedTile; };
+template <> struct TileOf<Q1_LAYOUT_COMPRESSED> { using T = CompressedTile<false>; };
+template <> struct TileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN> { using T = CompressedTile<true>; };
+
+// Cutoff in the units of the layout's ship_date column.
+template <int L>
+__device__ __forceinline__ int32_t column_cutoff(int32_t cutoff_day) {
+  if (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) {
+    const int64_t c = (int64_t)cutoff_day - kShipDateFrameOfReference;
+    return c < -1 ? -1 : (c > 65535 ? 65535 : (int32_t)c);
+  }
+  return cutoff_day;
+}
+
+// ------------------------------------------------------------------------------------
+// Per-lane partial table handed to the block merge: six base sums per group.
+//   F_QTY quantity (layout units), F_PRICE price, F_DISC discount, F_CNT count,
+//   F_DPROD sum of disc*price, F_CHARGE charge.  All exact in uint64 per lane.
+// ------------------------------------------------------------------------------------
+enum { F_QTY, F_PRICE, F_DISC, F_CNT, F_DPROD, F_CHARGE, NF };
+
+struct LaneTotals {
+  uint64_t v[kNumGroups][NF];
+};
+
+__device__ __forceinline__ uint32_t pack_small(const FastRow& r) {
+  return 1u | r.disc << 5 | r.qty << 13;
+}
+__device__ __forceinline__ uint32_t disc_prod(const FastRow& r) { return __umul24(r.disc, r.price); }
+__device__ __forceinline__ uint64_t fast_charge(const FastRow& r) {
+  const uint32_t f = __umul24(100u - r.disc, 100u + r.tax);  // <= 100*115, fits 24 bits
+  const uint64_t prod = (uint64_t)(f & 0xFFFFFFu) * (r.price & 0xFFFFFFu);
+  return prod;
+}
+
+// Q1_PLACE_IN_REGISTERS: narrow per-tile accumulators in registers; a group's block of
+// adds is skipped by the whole wavefront when no lane holds a row of that group.
+struct AccRegisters {
+  uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
+  uint64_t ch[kNumGroups], pr[kNumGroups], dp[kNumGroups];
+  uint32_t cnt[kNumGroups], dsc[kNumGroups], qty[kNumGroups];
+
+  __device__ __forceinline__ void init(char*) {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      ns[g] = np[g] = nd[g] = 0;
+      ch[g] = pr[g] = dp[g] = 0;
+      cnt[g] = dsc[g] = qty[g] = 0;
+    }
+  }
+  __device__ __forceinline__ void add(const FastRow& r) {
+    const uint32_t s = pack_small(r);
+    const uint32_t dprod = disc_prod(r);
+    const uint64_t c = fast_charge(r);
+    const uint64_t pass_mask = ballot(r.pass);
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      const bool mine = r.pass && r.gid == (uint32_t)g;
+      if (ballot(r.gid == (uint32_t)g) & pass_mask) {
+        // The empty asm keeps this a separate, wavefront-uniform branch (it cannot be
+        // folded into the lane mask), so a group no lane holds costs no adds at all.
+        __asm__ volatile("");
+        if (mine) {
+          ns[g] += s;
+          np[g] += r.price;
+          nd[g] += dprod;
+          ch[g] += c;
+        }
+      }
+    }
+  }
+  // Called after at most 16 rows: moves the narrow sums into the wide ones.
+  __device__ __forceinline__ void flush() {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      if (ballot(ns[g] != 0)) {
+        cnt[g] += ns[g] & 31u;
+        dsc[g] += (ns[g] >> 5) & 255u;
+        qty[g] += ns[g] >> 13;
+        pr[g] += np[g];
+        dp[g] += nd[g];
+        ns[g] = np[g] = nd[g] = 0;
+      }
+    }
+  }
+  __device__ __forceinline__ void totals(LaneTotals& t, char*) const {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      t.v[g][F_QTY] = qty[g];
+      t.v[g][F_PRICE] = pr[g];
+      t.v[g][F_DISC] = dsc[g];
+      t.v[g][F_CNT] = cnt[g];
+      t.v[g][F_DPROD] = dp[g];
+      t.v[g][F_CHARGE] = ch[g];
+    }
+  }
+  static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
+};
+
+// Q1_PLACE_IN_REGISTERS_PER_THREAD: the straightforward per-thread table: six full-width
+// sums per group in registers, every row updates its group through an unrolled
+// compare-select over all six groups (src/kernels/ht_in_registers_per_thread.cuh:54-64).
+struct AccPerThread {
+  uint64_t v[kNumGroups][NF];
+  __device__ __forceinline__ void init(char*) {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) v[g][f] = 0;
+  }
+  __device__ __forceinline__ void add(const FastRow& r) {
+    const uint64_t dprod = disc_prod(r);
+    const uint64_t c = fast_charge(r);
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      const bool m = r.pass && r.gid == (uint32_t)g;
+      v[g][F_QTY] += m ? r.qty : 0u;
+      v[g][F_PRICE] += m ? r.price : 0u;
+      v[g][F_DISC] += m ? r.disc : 0u;
+      v[g][F_CNT] += m ? 1u : 0u;
+      v[g][F_DPROD] += m ? dprod : 0u;
+      v[g][F_CHARGE] += m ? c : 0u;
+    }
+  }
+  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void totals(LaneTotals& t, char*) const {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) t.v[g][f] = v[g][f];
+  }
+  static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
+};
+
+// Packed (count | discount | quantity) in 64 bits for the memory-resident tables:
+// count bits [0,17), discount [17,37), quantity [37,64).  Exact while a lane's table
+// sees at most 16384 rows (tuples_per_thread is capped there).
+__device__ __forceinline__ uint64_t pack_small64(const FastRow& r) {
+  return 1ull | (uint64_t)r.disc << 17 | (uint64_t)r.qty << 37;
+}
+__device__ __forceinline__ void unpack_small64(uint64_t s, uint64_t* out) {
+  out[F_CNT] = s & ((1ull << 17) - 1);
+  out[F_DISC] = (s >> 17) & ((1ull << 20) - 1);
+  out[F_QTY] = s >> 37;
+}
+
+// Q1_PLACE_LOCAL_MEM: the per-thread table is indexed by the row's group at run time,
+// so it lives in the lane's private (scratch) memory; slot 7 absorbs non-counting rows.
+struct AccLocal {
+  volatile uint64_t t[8][4];  // [slot][small, price, dprod, charge]
+  __device__ __forceinline__ void init(char*) {
+    for (int g = 0; g < 8; g++)
+      for (int f = 0; f < 4; f++) t[g][f] = 0;
+  }
+  __device__ __forceinline__ void add(const FastRow& r) {
+    const uint32_t g = r.pass ? r.gid : 7u;  // slots 6 and 7 never reach the result
+    t[g][0] = t[g][0] + pack_small64(r);
+    t[g][1] = t[g][1] + r.price;
+    t[g][2] = t[g][2] + disc_prod(r);
+    t[g][3] = t[g][3] + fast_charge(r);
+  }
+  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
+    for (int g = 0; g < kNumGroups; g++) {
+      unpack_small64(t[g][0], o.v[g]);
+      o.v[g][F_PRICE] = t[g][1];
+      o.v[g][F_DPROD] = t[g][2];
+      o.v[g][F_CHARGE] = t[g][3];
+    }
+  }
+  static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
+};
+
+// Q1_PLACE_PER_THREAD_SHARED_MEM: each lane owns a table in LDS laid out
+// [field][group][thread] (consecutive lanes -> consecutive 8-byte words, so a wavefront's
+// update of any mix of groups is bank-conflict free); rows are added with return-less
+// LDS atomics, which need no VALU add and no read-back.
+struct AccShared {
+  uint64_t* tab;  // dynamic LDS: 4 * 6 * blockDim words
+  uint32_t stride;
+  __device__ __forceinline__ void init(char* lds) {
+    tab = reinterpret_cast<uint64_t*>(lds);
+    stride = blockDim.x;
+    for (int i = 0; i < 4 * kNumGroups; i++) tab[i * stride + threadIdx.x] = 0;
+  }
+  __device__ __forceinline__ void add(const FastRow& r) {
+    if (r.counts()) {
+      uint64_t* base = tab + r.gid * stride + threadIdx.x;
+      const uint32_t f = kNumGroups * stride;
+      __hip_atomic_fetch_add(base, pack_small64(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + f, (uint64_t)r.price, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + 2 * f, (uint64_t)disc_prod(r), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + 3 * f, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
+    const uint32_t f = kNumGroups * stride;
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      const uint64_t* base = tab + g * stride + threadIdx.x;
+      unpack_small64(base[0], o.v[g]);
+      o.v[g][F_PRICE] = base[f];
+      o.v[g][F_DPROD] = base[2 * f];
+      o.v[g][F_CHARGE] = base[3 * f];
+    }
+  }
+  static constexpr uint32_t lds_bytes(uint32_t threads) { return 4u * kNumGroups * 8u * threads; }
+};
+
+// Q1_PLACE_GLOBAL: one table per block in global memory (the kernel's workspace),
+// updated by every passing row with device-memory atomics.  The table is shared by all
+// lanes of the block, so it sees up to blockDim * 16384 rows: count and discount share one
+// word (count bits [0,24), discount [24,64)), the other sums have a word each.
+struct AccGlobal {
+  static constexpr int W = 5;  // words per group: count|disc, qty, price, dprod, charge
+  uint64_t* tab;               // [W][6] words of this block
+  __device__ __forceinline__ void init_global(uint64_t* ws) {
+    tab = ws + (uint64_t)blockIdx.x * (W * kNumGroups);
+    if (threadIdx.x < W * kNumGroups)
+      __hip_atomic_store(tab + threadIdx.x, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+  }
+  __device__ __forceinline__ void init(char*) {}
+  __device__ __forceinline__ void add(const FastRow& r) {
+    if (r.counts()) {
+      uint64_t* base = tab + r.gid;
+      atomicAdd((unsigned long long*)base, (unsigned long long)(1ull | (uint64_t)r.disc << 24));
+      atomicAdd((unsigned long long*)(base + kNumGroups), (unsigned long long)r.qty);
+      atomicAdd((unsigned long long*)(base + 2 * kNumGroups), (unsigned long long)r.price);
+      atomicAdd((unsigned long long*)(base + 3 * kNumGroups), (unsigned long long)disc_prod(r));
+      atomicAdd((unsigned long long*)(base + 4 * kNumGroups), (unsigned long long)fast_charge(r));
+    }
+  }
+  __device__ __forceinline__ void flush() {}
+  // The whole block's table is contributed by thread 0; the other lanes contribute zero.
+  __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
+    for (int g = 0; g < kNumGroups; g++)
+      for (int f = 0; f < NF; f++) o.v[g][f] = 0;
+    if (threadIdx.x == 0) {
+      for (int g = 0; g < kNumGroups; g++) {
+        auto ld = [&](int w) { return __hip_atomic_load(tab + w * kNumGroups + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+        const uint64_t cd = ld(0);
+        o.v[g][F_CNT] = cd & ((1ull << 24) - 1);
+        o.v[g][F_DISC] = cd >> 24;
+        o.v[g][F_QTY] = ld(1);
+        o.v[g][F_PRICE] = ld(2);
+        o.v[g][F_DPROD] = ld(3);
+        o.v[g][F_CHARGE] = ld(4);
+      }
+    }
+  }
+  static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
+};
+
+template <int P> struct AccOf;
+template <> struct AccOf<Q1_PLACE_IN_REGISTERS> { using A = AccRegisters; };
+template <> struct AccOf<Q1_PLACE_IN_REGISTERS_PER_THREAD> { using A = AccPerThread; };
+template <> struct AccOf<Q1_PLACE_LOCAL_MEM> { using A = AccLocal; };
+template <> struct AccOf<Q1_PLACE_PER_THREAD_SHARED_MEM> { using A = AccShared; };
+template <> struct AccOf<Q1_PLACE_GLOBAL> { using A = AccGlobal; };
+
+// ------------------------------------------------------------------------------------
+// Block merge and the single global merge per block.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// 128-bit signed add of (lo, hi) into a device (lo, hi) word pair with two atomics.
+__device__ __forceinline__ void atomic_add128(uint64_t* lo_p, uint64_t* hi_p, uint64_t lo, uint64_t hi) {
+  uint64_t carry = 0;
+  if (lo) {
+    const uint64_t old = atomicAdd((unsigned long long*)lo_p, (unsigned long long)lo);
+    carry = (old + lo) < old;
+  }
+  if (hi + carry) atomicAdd((unsigned long long*)hi_p, (unsigned long long)(hi + carry));
+}
+
+// Block-wide reduction of LaneTotals (exact: lane sums fit uint64, block sums are taken
+// in 128 bits) and the merge into `out`.  `red` is LDS scratch of
+// (blockDim/64) * 6 * NF uint64 words.  qty_scale turns layout units into x100.
+__device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggregates* out,
+                            uint64_t qty_scale) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+#pragma unroll
+  for (int g = 0; g < kNumGroups; g++) {
+    const bool any = ballot(t.v[g][F_CNT] != 0) != 0;
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const uint64_t s = any ? wave_sum_u64(t.v[g][f]) : 0;
+      if (lane == 0) red[(wave * kNumGroups + g) * NF + f] = s;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumGroups) {
+    const int g = threadIdx.x;
+    uint64_t lo[NF], hi[NF];
+    for (int f = 0; f < NF; f++) lo[f] = hi[f] = 0;
+    for (uint32_t w = 0; w < nwaves; w++) {
+      for (int f = 0; f < NF; f++) {
+        const uint64_t v = red[(w * kNumGroups + g) * NF + f];
+        lo[f] += v;
+        hi[f] += lo[f] < v;
+      }
+    }
+    if (lo[F_CNT] | hi[F_CNT]) {
+      atomicAdd((unsigned long long*)&out->sum_qty[g], (unsigned long long)(lo[F_QTY] * qty_scale));
+      atomicAdd((unsigned long long*)&out->sum_base_price[g], (unsigned long long)lo[F_PRICE]);
+      atomicAdd((unsigned long long*)&out->sum_disc[g], (unsigned long long)lo[F_DISC]);
+      atomicAdd((unsigned long long*)&out->count[g], (unsigned long long)lo[F_CNT]);
+      // sum_disc_price = 100 * sum_price - sum(disc * price), in 128 bits
+      const unsigned __int128 p100 = ((unsigned __int128)hi[F_PRICE] << 64 | lo[F_PRICE]) * 100u;
+      const unsigned __int128 dprod = (unsigned __int128)hi[F_DPROD] << 64 | lo[F_DPROD];
+      const unsigned __int128 dp = p100 - dprod;
+      atomic_add128(&out->sum_disc_price_lo[g], &out->sum_disc_price_hi[g], (uint64_t)dp,
+                    (uint64_t)(dp >> 64));
+      atomic_add128(&out->sum_charge_lo[g], &out->sum_charge_hi[g], lo[F_CHARGE], hi[F_CHARGE]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// The fast scan kernel: layout L, placement P.
+// ------------------------------------------------------------------------------------
+template <int L, int P>
+__global__ void __launch_bounds__(kMaxBlock)
+scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
+  using Tile = typename TileOf<L>::T;
+  using Acc = typename AccOf<P>::A;
+  constexpr int TILE = Tile::TILE;
+  constexpr int FLUSH = 16 / TILE;  // tiles between narrow-accumulator flushes
+  extern __shared__ uint64_t lds[];
+
+  const int32_t cut = column_cutoff<L>(a.cutoff_day);
+  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
+  const uint64_t base = first_tile + (uint64_t)blockIdx.x * blockDim.x * tiles_per_thread;
+
+  Acc acc;
+  if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
+  acc.init(reinterpret_cast<char*>(lds));
+
+  for (uint32_t k = 0; k < tiles_per_thread; k++) {
+    const uint64_t t = base + (uint64_t)k * blockDim.x + threadIdx.x;
+    if (t < end_tile) {
+      Tile tile;
+      const bool edge = t * TILE < a.row_begin || (t + 1) * TILE > a.row_end;
+      if (!edge) {
+        tile.load(a.cols, t);
+#pragma unroll
+        for (int j = 0; j < TILE; j++) acc.add(tile.template fast<false>(j, cut, 0));
+      } else {
+        const uint32_t valid = valid_mask<TILE>(t, a.row_begin, a.row_end);
+        tile.load_edge(a.cols, t, valid);
+#pragma unroll
+        for (int j = 0; j < TILE; j++) acc.add(tile.template fast<true>(j, cut, valid));
+      }
+    }
+    if ((k + 1) % FLUSH == 0) acc.flush();
+  }
+  acc.flush();
+
+  if constexpr (P == Q1_PLACE_GLOBAL) {
+    __threadfence();
+    __syncthreads();
+  }
+  LaneTotals tot;
+  acc.totals(tot, reinterpret_cast<char*>(lds));
+  // The reduction scratch follows the placement's own LDS table.
+  uint64_t* red = lds + Acc::lds_bytes(blockDim.x) / 8;
+  __syncthreads();
+  const uint64_t qty_scale =
+      (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
+  block_merge(tot, red, a.out, qty_scale);
+}
+
+// ------------------------------------------------------------------------------------
+// The general (64-bit) scan kernel: per-lane register table, wrapping int64 per-row
+// arithmetic, 128-bit signed sums of disc_price and charge.
+// ------------------------------------------------------------------------------------
+struct Wide128 {
+  uint64_t lo;
+  uint64_t hi;
+  __device__ __forceinline__ void add64s(int64_t v) {
+    const uint64_t nlo = lo + (uint64_t)v;
+    hi += (uint64_t)(v >> 63) + (nlo < lo);
+    lo = nlo;
+  }
+};
+
+template <int L>
+__global__ void __launch_bounds__(kMaxBlock)
+scan_general(ScanArgs a, uint32_t tiles_per_thread) {
+  using Tile = typename TileOf<L>::T;
+  constexpr int TILE = Tile::TILE;
+  extern __shared__ uint64_t lds[];
+
+  const int32_t cut = column_cutoff<L>(a.cutoff_day);
+  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
+  const uint64_t base = first_tile + (uint64_t)blockIdx.x * blockDim.x * tiles_per_thread;
+
+  uint64_t qty[kNumGroups], price[kNumGroups], disc[kNumGroups], cnt[kNumGroups];
+  Wide128 dp[kNumGroups], ch[kNumGroups];
+#pragma unroll
+  for (int g = 0; g < kNumGroups; g++) {
+    qty[g] = price[g] = disc[g] = cnt[g] = 0;
+    dp[g].lo = dp[g].hi = ch[g].lo = ch[g].hi = 0;
+  }
+
+  for (uint32_t k = 0; k < tiles_per_thread; k++) {
+    const uint64_t t = base + (uint64_t)k * blockDim.x + threadIdx.x;
+    if (t >= end_tile) break;
+    Tile tile;
+    uint32_t valid = (1u << TILE) - 1;
+    if (t * TILE < a.row_begin || (t + 1) * TILE > a.row_end) {
+      valid = valid_mask<TILE>(t, a.row_begin, a.row_end);
+      tile.load_edge(a.cols, t, valid);
+    } else {
+      tile.load(a.cols, t);
+    }
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      const WideRow r = tile.wide(j, cut, valid);
+      const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
+      const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
+      const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
+#pragma unroll
+      for (int g = 0; g < kNumGroups; g++) {
+        if (r.gid == (uint32_t)g) {
+          qty[g] += (uint64_t)r.qty;
+          price[g] += (uint64_t)r.price;
+          disc[g] += (uint64_t)r.disc;
+          cnt[g] += 1;
+          dp[g].add64s(disc_price);
+          ch[g].add64s(charge);
+        }
+      }
+    }
+  }
+
+  // Block merge in 128 bits for every field.
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+  constexpr int NW = 8;  // qty, price, disc, cnt, dp.lo, dp.hi, ch.lo, ch.hi
+  uint64_t* red = lds;   // nwaves * 6 * NW * 2 words
+#pragma unroll
+  for (int g = 0; g < kNumGroups; g++) {
+    uint64_t s64[4] = {qty[g], price[g], disc[g], cnt[g]};
+    Wide128 w[2] = {dp[g], ch[g]};
+    if (ballot(cnt[g] != 0)) {
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int f = 0; f < 4; f++) s64[f] += __shfl_xor(s64[f], m, 64);
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+          const uint64_t olo = __shfl_xor(w[f].lo, m, 64), ohi = __shfl_xor(w[f].hi, m, 64);
+          const uint64_t nlo = w[f].lo + olo;
+          w[f].hi += ohi + (nlo < olo);
+          w[f].lo = nlo;
+        }
+      }
+    } else {
+      s64[0] = s64[1] = s64[2] = s64[3] = 0;
+      w[0].lo = w[0].hi = w[1].lo = w[1].hi = 0;
+    }
+    if (lane == 0) {
+      uint64_t* r = red + (wave * kNumGroups + g) * NW;
+      r[0] = s64[0]; r[1] = s64[1]; r[2] = s64[2]; r[3] = s64[3];
+      r[4] = w[0].lo; r[5] = w[0].hi; r[6] = w[1].lo; r[7] = w[1].hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumGroups) {
+    const int g = threadIdx.x;
+    uint64_t s[4] = {0, 0, 0, 0};
+    Wide128 w[2] = {{0, 0}, {0, 0}};
+    for (uint32_t v = 0; v < nwaves; v++) {
+      const uint64_t* r = red + (v * kNumGroups + g) * NW;
+      for (int f = 0; f < 4; f++) s[f] += r[f];
+      for (int f = 0; f < 2; f++) {
+        const uint64_t nlo = w[f].lo + r[4 + 2 * f];
+        w[f].hi += r[5 + 2 * f] + (nlo < w[f].lo);
+        w[f].lo = nlo;
+      }
+    }
+    if (s[3]) {
+      const uint64_t qs =
+          (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
+      atomicAdd((unsigned long long*)&a.out->sum_qty[g], (unsigned long long)(s[0] * qs));
+      atomicAdd((unsigned long long*)&a.out->sum_base_price[g], (unsigned long long)s[1]);
+      atomicAdd((unsigned long long*)&a.out->sum_disc[g], (unsigned long long)s[2]);
+      atomicAdd((unsigned long long*)&a.out->count[g], (unsigned long long)s[3]);
+      atomic_add128(&a.out->sum_disc_price_lo[g], &a.out->sum_disc_price_hi[g], w[0].lo, w[0].hi);
+      atomic_add128(&a.out->sum_charge_lo[g], &a.out->sum_charge_hi[g], w[1].lo, w[1].hi);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Utility kernels: zero fill, generator, domain statistics, pushdown producer.
+// ------------------------------------------------------------------------------------
+__global__ void zero_words(uint64_t* p, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// One thread per 32-row group writes the rows' values and the packed words of the group.
+template <int L>
+__global__ void generate_rows(uint64_t seed, uint64_t rows, q1_columns c) {
+  const uint64_t grp = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t r0 = grp * 32;
+  if (r0 >= rows) return;
+  const uint64_t r1 = r0 + 32 < rows ? r0 + 32 : rows;
+  uint32_t rfw[2] = {0, 0}, lsw = 0;
+  for (uint64_t r = r0; r < r1; r++) {
+    const GenRow g = gen_row(seed, r);
+    const uint32_t j = (uint32_t)(r - r0);
+    if (L == Q1_LAYOUT_REFERENCE) {
+      ((int32_t*)c.ship_date)[r] = g.ship_date;
+      ((int64_t*)c.quantity)[r] = (int64_t)g.quantity * 100;
+      ((int64_t*)c.extended_price)[r] = g.extended_price;
+      ((int64_t*)c.discount)[r] = g.discount;
+      ((int64_t*)c.tax)[r] = g.tax;
+      ((char*)c.return_flag)[r] = g.return_flag;
+      ((char*)c.line_status)[r] = g.line_status;
+    } else if (L == Q1_LAYOUT_UNCOMPRESSED) {
+      ((int32_t*)c.ship_date)[r] = g.ship_date;
+      ((int32_t*)c.quantity)[r] = g.quantity * 100;
+      ((int64_t*)c.extended_price)[r] = g.extended_price;
+      ((int8_t*)c.discount)[r] = (int8_t)g.discount;
+      ((int8_t*)c.tax)[r] = (int8_t)g.tax;
+      ((char*)c.return_flag)[r] = g.return_flag;
+      ((char*)c.line_status)[r] = g.line_status;
+    } else {
+      ((uint16_t*)c.ship_date)[r] = (uint16_t)(g.ship_date - kShipDateFrameOfReference);
+      ((uint8_t*)c.quantity)[r] = (uint8_t)g.quantity;
+      ((uint32_t*)c.extended_price)[r] = (uint32_t)g.extended_price;
+      ((uint8_t*)c.discount)[r] = (uint8_t)g.discount;
+      ((uint8_t*)c.tax)[r] = (uint8_t)g.tax;
+      rfw[j >> 4] |= encode_return_flag((uint32_t)g.return_flag) << (2 * (j & 15));
+      lsw |= encode_line_status((uint32_t)g.line_status) << j;
+    }
+  }
+  if (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) {
+    uint32_t* rf = (uint32_t*)c.return_flag;
+    rf[2 * grp] = rfw[0];
+    if (r0 + 16 < rows) rf[2 * grp + 1] = rfw[1];
+    ((uint32_t*)c.line_status)[grp] = lsw;
+  }
+}
+
+// min/max of discount, tax, quantity, price: out[0..7] = {min,max} x {disc,tax,qty,price}.
+template <int L>
+__global__ void domain_stats(q1_columns c, uint64_t rows, int64_t* out) {
+  int64_t mn[4] = {INT64_MAX, INT64_MAX, INT64_MAX, INT64_MAX};
+  int64_t mx[4] = {INT64_MIN, INT64_MIN, INT64_MIN, INT64_MIN};
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    int64_t v[4];
+    if (L == Q1_LAYOUT_REFERENCE) {
+      v[0] = ((const int64_t*)c.discount)[r];
+      v[1] = ((const int64_t*)c.tax)[r];
+      v[2] = ((const int64_t*)c.quantity)[r];
+      v[3] = ((const int64_t*)c.extended_price)[r];
+    } else if (L == Q1_LAYOUT_UNCOMPRESSED) {
+      v[0] = ((const int8_t*)c.discount)[r];
+      v[1] = ((const int8_t*)c.tax)[r];
+      v[2] = ((const int32_t*)c.quantity)[r];
+      v[3] = ((const int64_t*)c.extended_price)[r];
+    } else {
+      v[0] = ((const uint8_t*)c.discount)[r];
+      v[1] = ((const uint8_t*)c.tax)[r];
+      v[2] = ((const uint8_t*)c.quantity)[r];
+      v[3] = ((const uint32_t*)c.extended_price)[r];
+    }
+    for (int i = 0; i < 4; i++) {
+      mn[i] = v[i] < mn[i] ? v[i] : mn[i];
+      mx[i] = v[i] > mx[i] ? v[i] : mx[i];
+    }
+  }
+  for (int i = 0; i < 4; i++) {
+    for (int m = 32; m >= 1; m >>= 1) {
+      const int64_t omn = __shfl_xor(mn[i], m, 64), omx = __shfl_xor(mx[i], m, 64);
+      mn[i] = omn < mn[i] ? omn : mn[i];
+      mx[i] = omx > mx[i] ? omx : mx[i];
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    for (int i = 0; i < 4; i++) {
+      atomicMin((long long*)&out[2 * i], (long long)mn[i]);
+      atomicMax((long long*)&out[2 * i + 1], (long long)mx[i]);
+    }
+  }
+}
+
+// One thread per 32 rows (src/main.cu:362-389 semantics, runtime cutoff).
+__global__ void build_filter(const uint16_t* ship, uint64_t rows, int32_t cut, uint32_t* filter) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t r0 = w * 32;
+  if (r0 >= rows) return;
+  const uint64_t n = rows - r0 < 32 ? rows - r0 : 32;
+  uint32_t bits = 0;
+  for (uint32_t j = 0; j < n; j++) bits |= (uint32_t)((int32_t)ship[r0 + j] <= cut) << j;
+  filter[w] = bits;
+}
+
+// ------------------------------------------------------------------------------------
+// The batched persistent kernel.
+// ------------------------------------------------------------------------------------
+template <int L, int P>
+__global__ void __launch_bounds__(kMaxBlock)
+batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
+  using Tile = typename TileOf<L>::T;
+  using Acc = typename AccOf<P>::A;
+  constexpr int TILE = Tile::TILE;
+  constexpr int FLUSH = 16 / TILE;
+  constexpr uint64_t kMaxTilesPerLane = 16384 / TILE;  // keeps lane partials exact
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t first_req;
+
+  const uint64_t n = hdr->n, total = hdr->total_tiles;
+  const BatchDesc* descs = reinterpret_cast<const BatchDesc*>(hdr + 1);
+  const uint64_t span = (total + gridDim.x - 1) / gridDim.x;
+  uint64_t pos = (uint64_t)blockIdx.x * span;
+  const uint64_t stop = pos + span < total ? pos + span : total;
+  if (pos >= stop) return;  // uniform for the whole block
+  if (threadIdx.x == 0) {
+    // last request whose first tile is <= pos
+    uint64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (descs[mid].tile_begin <= pos) lo = mid; else hi = mid;
+    }
+    first_req = (uint32_t)lo;
+  }
+  __syncthreads();
+  uint64_t r = first_req;
+  uint64_t* red = lds + Acc::lds_bytes(blockDim.x) / 8;
+  while (pos < stop && r < n) {
+    const BatchDesc d = descs[r];
+    const uint64_t first_tile = d.row_begin / TILE;
+    const uint64_t req_tiles = (d.row_end + TILE - 1) / TILE - first_tile;
+    const uint64_t req_end = d.tile_begin + req_tiles;
+    if (req_tiles == 0 || pos >= req_end) {  // empty request or already past it
+      r++;
+      continue;
+    }
+    uint64_t seg_end = stop < req_end ? stop : req_end;
+    const uint64_t cap = pos + (uint64_t)blockDim.x * kMaxTilesPerLane;
+    if (seg_end > cap) seg_end = cap;
+    const int32_t cut = column_cutoff<L>(d.cutoff_day);
+    Acc acc;
+    acc.init(reinterpret_cast<char*>(lds));
+    uint32_t k = 0;
+    for (uint64_t b = pos; b < seg_end; b += blockDim.x, k++) {
+      const uint64_t g = b + threadIdx.x;
+      if (g < seg_end) {
+        const uint64_t t = first_tile + (g - d.tile_begin);
+        Tile tile;
+        const bool edge = t * TILE < d.row_begin || (t + 1) * TILE > d.row_end;
+        if (!edge) {
+          tile.load(d.cols, t);
+#pragma unroll
+          for (int j = 0; j < TILE; j++) acc.add(tile.template fast<false>(j, cut, 0));
+        } else {
+          const uint32_t valid = valid_mask<TILE>(t, d.row_begin, d.row_end);
+          tile.load_edge(d.cols, t, valid);
+#pragma unroll
+          for (int j = 0; j < TILE; j++) acc.add(tile.template fast<true>(j, cut, valid));
+        }
+      }
+      if ((k + 1) % FLUSH == 0) acc.flush();
+    }
+    acc.flush();
+    LaneTotals tot;
+    acc.totals(tot, reinterpret_cast<char*>(lds));
+    __syncthreads();
+    const uint64_t qty_scale =
+        (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
+    block_merge(tot, red, &outs[r], qty_scale);
+    __syncthreads();  // red and the placement's LDS table are reused by the next segment
+    pos = seg_end;
+    if (pos >= req_end) r++;
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// Dispatch tables
+// ------------------------------------------------------------------------------------
+using FastFn = void (*)(ScanArgs, uint32_t, uint64_t*);
+using GenFn = void (*)(ScanArgs, uint32_t);
+
+template <int L>
+constexpr FastFn fast_fn(int p) {
+  return p == Q1_PLACE_IN_REGISTERS ? scan_fast<L, Q1_PLACE_IN_REGISTERS>
+       : p == Q1_PLACE_IN_REGISTERS_PER_THREAD ? scan_fast<L, Q1_PLACE_IN_REGISTERS_PER_THREAD>
+       : p == Q1_PLACE_LOCAL_MEM ? scan_fast<L, Q1_PLACE_LOCAL_MEM>
+       : p == Q1_PLACE_PER_THREAD_SHARED_MEM ? scan_fast<L, Q1_PLACE_PER_THREAD_SHARED_MEM>
+       : scan_fast<L, Q1_PLACE_GLOBAL>;
+}
+
+FastFn fast_kernel(q1_layout l, q1_placement p) {
+  switch (l) {
+    case Q1_LAYOUT_REFERENCE: return fast_fn<Q1_LAYOUT_REFERENCE>(p);
+    case Q1_LAYOUT_UNCOMPRESSED: return fast_fn<Q1_LAYOUT_UNCOMPRESSED>(p);
+    case Q1_LAYOUT_COMPRESSED: return fast_fn<Q1_LAYOUT_COMPRESSED>(p);
+    default: return fast_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(p);
+  }
+}
+
+GenFn general_kernel(q1_layout l) {
+  switch (l) {
+    case Q1_LAYOUT_REFERENCE: return scan_general<Q1_LAYOUT_REFERENCE>;
+    case Q1_LAYOUT_UNCOMPRESSED: return scan_general<Q1_LAYOUT_UNCOMPRESSED>;
+    case Q1_LAYOUT_COMPRESSED: return scan_general<Q1_LAYOUT_COMPRESSED>;
+    default: return scan_general<Q1_LAYOUT_COMPRESSED_PUSHDOWN>;
+  }
+}
+
+uint32_t tile_rows(q1_layout l) {
+  return l == Q1_LAYOUT_REFERENCE ? 4u : (l == Q1_LAYOUT_UNCOMPRESSED ? 8u : 16u);
+}
+
+q1_status hip_status(hipError_t e) { return e == hipSuccess ? Q1_OK : Q1_EDEVICE; }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------
+// Host-side launch helpers
+// ------------------------------------------------------------------------------------
+bool domain_is_tpch(q1_layout layout, const Domain& d) {
+  const int64_t max_qty =
+      (layout == Q1_LAYOUT_COMPRESSED || layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? 63 : 6300;
+  return d.min_disc >= 0 && d.max_disc <= 15 && d.min_tax >= 0 && d.max_tax <= 15 && d.min_qty >= 0 &&
+         d.max_qty <= max_qty && d.min_price >= 0 && d.max_price < (1ll << 24);
+}
+
+q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* launch,
+                     uint64_t row_begin, uint64_t row_end, LaunchShape* shape, const char** why) {
+  const uint32_t trows = tile_rows(layout);
+  uint32_t threads = launch && launch->threads_per_block ? launch->threads_per_block : 256;
+  uint32_t tpt = launch && launch->tuples_per_thread ? launch->tuples_per_thread : 1024;
+  if (threads % 64 != 0 || threads == 0 || threads > (uint32_t)kMaxBlock) {
+    *why = "threads_per_block must be a multiple of 64 in [64, 256]";
+    return Q1_EINVAL;
+  }
+  if (tpt % 16 != 0 || tpt == 0 || tpt > 16384) {
+    *why = "tuples_per_thread must be a multiple of 16 in [16, 16384]";
+    return Q1_EINVAL;
+  }
+  if ((int)placement < 0 || placement > Q1_PLACE_GLOBAL) {
+    *why = "unknown placement";
+    return Q1_EINVAL;
+  }
+  const uint32_t tiles_per_thread = tpt / trows;
+  const uint64_t first = row_begin / trows, end = (row_end + trows - 1) / trows;
+  const uint64_t tiles = end > first ? end - first : 0;
+  const uint64_t per_block = (uint64_t)threads * tiles_per_thread;
+  uint64_t blocks = (tiles + per_block - 1) / per_block;
+  if (launch && launch->num_blocks) {
+    // A caller-chosen grid must still cover the range with the chosen tuples per thread.
+    if ((uint64_t)launch->num_blocks < blocks) {
+      *why = "num_blocks too small to cover the rows with tuples_per_thread";
+      return Q1_EINVAL;
+    }
+    blocks = launch->num_blocks;
+  }
+  if (blocks > 0x7FFFFFFFull) {
+    *why = "grid too large";
+    return Q1_EINVAL;
+  }
+  shape->blocks = (uint32_t)blocks;
+  shape->threads = threads;
+  shape->tiles_per_thread = tiles_per_thread;
+  return Q1_OK;
+}
+
+uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape) {
+  return placement == Q1_PLACE_GLOBAL ? (uint64_t)shape.blocks * AccGlobal::W * kNumGroups * 8 : 0;
+}
+
+q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
+                      const LaunchShape& shape, void* workspace, void* stream) {
+  if (shape.blocks == 0) return Q1_OK;
+  const hipStream_t s = (hipStream_t)stream;
+  const uint32_t nwaves = shape.threads / 64;
+  if (fast) {
+    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared::lds_bytes(shape.threads) : 0;
+    lds += nwaves * kNumGroups * NF * 8;
+    hipLaunchKernelGGL(fast_kernel(layout, placement), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
+                       shape.tiles_per_thread, (uint64_t*)workspace);
+  } else {
+    const uint32_t lds = nwaves * kNumGroups * 8 * 8;
+    hipLaunchKernelGGL(general_kernel(layout), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
+                       shape.tiles_per_thread);
+  }
+  return hip_status(hipGetLastError());
+}
+
+q1_status launch_zero(q1_device_aggregates* table, uint32_t count, void* stream) {
+  const uint64_t n = (uint64_t)count * (sizeof(q1_device_aggregates) / 8);
+  if (n == 0) return Q1_OK;
+  hipLaunchKernelGGL(zero_words, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     (uint64_t*)table, n);
+  return hip_status(hipGetLastError());
+}
+
+q1_status launch_generate(q1_layout layout, uint64_t seed, uint64_t rows, const q1_columns& cols,
+                          void* stream) {
+  if (rows == 0) return Q1_OK;
+  const uint64_t groups = (rows + 31) / 32;
+  const dim3 grid((uint32_t)((groups + 255) / 256)), block(256);
+  const hipStream_t s = (hipStream_t)stream;
+  switch (layout) {
+    case Q1_LAYOUT_REFERENCE: hipLaunchKernelGGL(generate_rows<Q1_LAYOUT_REFERENCE>, grid, block, 0, s, seed, rows, cols); break;
+    case Q1_LAYOUT_UNCOMPRESSED: hipLaunchKernelGGL(generate_rows<Q1_LAYOUT_UNCOMPRESSED>, grid, block, 0, s, seed, rows, cols); break;
+    default: hipLaunchKernelGGL(generate_rows<Q1_LAYOUT_COMPRESSED>, grid, block, 0, s, seed, rows, cols); break;
+  }
+  return hip_status(hipGetLastError());
+}
+
+q1_status launch_domain(q1_layout layout, const q1_columns& cols, uint64_t rows, int64_t* out_dev,
+                        void* stream) {
+  if (rows == 0) return Q1_OK;
+  uint64_t blocks = (rows + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  const dim3 grid((uint32_t)blocks), block(256);
+  const hipStream_t s = (hipStream_t)stream;
+  switch (layout) {
+    case Q1_LAYOUT_REFERENCE: hipLaunchKernelGGL(domain_stats<Q1_LAYOUT_REFERENCE>, grid, block, 0, s, cols, rows, out_dev); break;
+    case Q1_LAYOUT_UNCOMPRESSED: hipLaunchKernelGGL(domain_stats<Q1_LAYOUT_UNCOMPRESSED>, grid, block, 0, s, cols, rows, out_dev); break;
+    default: hipLaunchKernelGGL(domain_stats<Q1_LAYOUT_COMPRESSED>, grid, block, 0, s, cols, rows, out_dev); break;
+  }
+  return hip_status(hipGetLastError());
+}
+
+q1_status launch_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutoff_day, uint32_t* filter,
+                              void* stream) {
+  if (rows == 0) return Q1_OK;
+  const int64_t c = (int64_t)cutoff_day - kShipDateFrameOfReference;
+  const int32_t cut = c < -1 ? -1 : (c > 65535 ? 65535 : (int32_t)c);
+  const uint64_t words = (rows + 31) / 32;
+  hipLaunchKernelGGL(build_filter, dim3((uint32_t)((words + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                     ship, rows, cut, filter);
+  return hip_status(hipGetLastError());
+}
+
+
+uint32_t layout_tile_rows(q1_layout layout) { return tile_rows(layout); }
+
+template <int L>
+static void* batch_fn(q1_placement p) {
+  return p == Q1_PLACE_PER_THREAD_SHARED_MEM ? (void*)batch_scan<L, Q1_PLACE_PER_THREAD_SHARED_MEM>
+                                             : (void*)batch_scan<L, Q1_PLACE_IN_REGISTERS>;
+}
+
+q1_status launch_batch(q1_layout layout, q1_placement placement, const BatchHeader* buf,
+                       q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream) {
+  if (placement != Q1_PLACE_PER_THREAD_SHARED_MEM) placement = Q1_PLACE_IN_REGISTERS;
+  uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared::lds_bytes(threads) : 0;
+  lds += threads / 64 * kNumGroups * NF * 8;
+  void* fn;
+  switch (layout) {
+    case Q1_LAYOUT_REFERENCE: fn = batch_fn<Q1_LAYOUT_REFERENCE>(placement); break;
+    case Q1_LAYOUT_UNCOMPRESSED: fn = batch_fn<Q1_LAYOUT_UNCOMPRESSED>(placement); break;
+    case Q1_LAYOUT_COMPRESSED: fn = batch_fn<Q1_LAYOUT_COMPRESSED>(placement); break;
+    default: fn = batch_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(placement); break;
+  }
+  void* args[] = {(void*)&buf, (void*)&outs};
+  return hip_status(hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, lds, (hipStream_t)stream));
+}
+
+}  // namespace q1

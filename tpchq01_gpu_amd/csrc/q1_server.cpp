@@ -1,0 +1,443 @@
+// Serving: continuous batching of Q1 requests over HBM-resident tables.
+//
+//   submit() -> pending queue (bounded: Q1_EBUSY when full)
+//   scheduler thread: whenever an in-flight slot is free, takes up to max_batch pending
+//     requests of one layout (waiting at most batch_window_us for more while other batches
+//     are still running), admits them while the device memory of in-flight request state
+//     stays within memory_budget, and launches them as one batch on the slot's stream
+//     (a hipGraph per slot: descriptor copy, zero fill kernel, batched kernel, result copy)
+//   one completion thread per slot: waits for the slot's batch, publishes every result
+//     with its latency, frees the slot.
+// Batches on different slots run concurrently on the device, so requests that arrive while
+// others are on the GPU start as soon as a slot frees, not when the device drains.
+// Requests on tables outside the TPC-H value domain run alone through the 64-bit kernel.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "q1_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Pending {
+  uint64_t ticket;
+  q1_request req;
+  Clock::time_point submitted;
+};
+
+struct Finished {
+  q1_status status = Q1_OK;
+  q1_result result{};
+  q1_request_stats stats{};
+};
+
+uint64_t request_state_bytes() { return sizeof(q1::BatchDesc) + sizeof(q1_device_aggregates); }
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  uint32_t capacity = 0;
+  q1_layout layout = Q1_LAYOUT_COMPRESSED;
+  bool single = false;           // a lone request through the 64-bit scan kernel
+  q1::BatchHeader* h_buf = nullptr;
+  q1::BatchHeader* d_buf = nullptr;
+  q1_device_aggregates* h_out = nullptr;
+  q1_device_aggregates* d_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<Pending> batch;
+  Clock::time_point started;
+  uint32_t batch_id = 0;
+  bool busy = false;
+  bool launched = false;         // the done event has been recorded
+  std::thread waiter;
+};
+
+}  // namespace
+
+struct q1_server {
+  q1_ctx* ctx = nullptr;
+  q1_server_config cfg{};
+  q1_placement placement = Q1_PLACE_IN_REGISTERS;
+  uint32_t blocks = 0;
+  std::mutex m;
+  std::condition_variable cv_sched, cv_done, cv_slot;
+  std::deque<Pending> queue;
+  std::unordered_map<uint64_t, Finished> finished;
+  std::unordered_map<uint64_t, bool> inflight;  // tickets on the device
+  uint64_t next_ticket = 1;
+  uint64_t bytes_in_flight = 0;
+  uint32_t batches = 0;
+  bool stop = false;
+  std::vector<Slot> slots;
+  std::thread scheduler;
+};
+
+namespace {
+
+void free_slot_buffers(Slot& s) {
+  if (s.exec) (void)hipGraphExecDestroy(s.exec);
+  if (s.graph) (void)hipGraphDestroy(s.graph);
+  if (s.h_buf) (void)hipHostFree(s.h_buf);
+  if (s.h_out) (void)hipHostFree(s.h_out);
+  if (s.d_buf) (void)hipFree(s.d_buf);
+  if (s.d_out) (void)hipFree(s.d_out);
+  s.exec = nullptr;
+  s.graph = nullptr;
+  s.h_buf = s.d_buf = nullptr;
+  s.h_out = s.d_out = nullptr;
+  s.capacity = 0;
+}
+
+// (Re)builds a slot's buffers and graph for `cap` requests of `layout`.
+q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
+  if (s.capacity == cap && s.layout == layout && s.exec) return Q1_OK;
+  free_slot_buffers(s);
+  const uint64_t bb = sizeof(q1::BatchHeader) + (uint64_t)cap * sizeof(q1::BatchDesc);
+  const uint64_t ob = (uint64_t)cap * sizeof(q1_device_aggregates);
+  if (hipHostMalloc(&s.h_buf, bb, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(&s.h_out, ob, hipHostMallocDefault) != hipSuccess || hipMalloc(&s.d_buf, bb) != hipSuccess ||
+      hipMalloc(&s.d_out, ob) != hipSuccess) {
+    free_slot_buffers(s);
+    return Q1_ENOMEM;
+  }
+  std::memset(s.h_buf, 0, bb);
+  q1_status st = Q1_OK;
+  if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) st = Q1_EDEVICE;
+  if (st == Q1_OK && hipMemcpyAsync(s.d_buf, s.h_buf, bb, hipMemcpyHostToDevice, s.stream) != hipSuccess) st = Q1_EDEVICE;
+  if (st == Q1_OK) st = q1::launch_zero(s.d_out, cap, s.stream);
+  if (st == Q1_OK) st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, sv->blocks, 256, s.stream);
+  if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+    st = Q1_EDEVICE;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s.stream, &g);
+  if (st != Q1_OK || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    free_slot_buffers(s);
+    return Q1_EDEVICE;
+  }
+  s.graph = g;
+  if (hipGraphInstantiate(&s.exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    free_slot_buffers(s);
+    return Q1_EDEVICE;
+  }
+  s.capacity = cap;
+  s.layout = layout;
+  return Q1_OK;
+}
+
+// Launches the slot's batch; called with the server lock released.
+q1_status launch_slot(q1_server* sv, Slot& s) {
+  const uint32_t n = (uint32_t)s.batch.size();
+  const q1_table* t0 = s.batch[0].req.table;
+  if (s.single) {
+    // One request on a table outside the TPC-H domain: the 64-bit scan kernel.
+    q1_status st = prepare_slot(sv, s, 16, t0->layout);
+    if (st != Q1_OK) return st;
+    const q1_request& r = s.batch[0].req;
+    q1::LaunchShape shape;
+    const char* why = "";
+    q1_launch l{256, 1024, 0, 0};
+    st = q1::make_shape(Q1_PLACE_IN_REGISTERS, t0->layout, &l, r.row_begin, r.row_begin + r.row_count, &shape, &why);
+    if (st != Q1_OK) return st;
+    q1::ScanArgs a;
+    a.cols = t0->dev;
+    a.row_begin = r.row_begin;
+    a.row_end = r.row_begin + r.row_count;
+    a.cutoff_day = t0->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t0->filter_cutoff : r.cutoff_day;
+    a.out = s.d_out;
+    st = q1::launch_zero(s.d_out, 1, s.stream);
+    if (st == Q1_OK) st = q1::launch_scan(Q1_PLACE_IN_REGISTERS, t0->layout, false, a, shape, nullptr, s.stream);
+    if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,
+                                      s.stream) != hipSuccess)
+      st = Q1_EDEVICE;
+    if (st == Q1_OK && hipEventRecord(s.done, s.stream) != hipSuccess) st = Q1_EDEVICE;
+    return st;
+  }
+  uint32_t cap = 16;
+  while (cap < n) cap <<= 1;
+  q1_status st = prepare_slot(sv, s, cap, t0->layout);
+  if (st != Q1_OK) return st;
+  const uint64_t trows = q1::layout_tile_rows(t0->layout);
+  q1::BatchDesc* d = reinterpret_cast<q1::BatchDesc*>(s.h_buf + 1);
+  uint64_t tiles = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const q1_request& r = s.batch[i].req;
+    d[i].cols = r.table->dev;
+    d[i].row_begin = r.row_begin;
+    d[i].row_end = r.row_begin + r.row_count;
+    d[i].tile_begin = tiles;
+    d[i].cutoff_day = r.table->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? r.table->filter_cutoff : r.cutoff_day;
+    d[i].pad = 0;
+    if (r.row_count) tiles += (d[i].row_end + trows - 1) / trows - d[i].row_begin / trows;
+  }
+  s.h_buf->n = n;
+  s.h_buf->total_tiles = tiles;
+  if (hipGraphLaunch(s.exec, s.stream) != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess)
+    return Q1_EDEVICE;
+  return Q1_OK;
+}
+
+void publish(q1_server* sv, Slot& s, q1_status st) {
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> g(sv->m);
+  for (size_t i = 0; i < s.batch.size(); i++) {
+    Finished f;
+    f.status = st;
+    if (st == Q1_OK) q1_aggregates_to_result(&s.h_out[i], &f.result);
+    f.stats.submit_to_start_us =
+        std::chrono::duration<double, std::micro>(s.started - s.batch[i].submitted).count();
+    f.stats.latency_us = std::chrono::duration<double, std::micro>(now - s.batch[i].submitted).count();
+    f.stats.batch_size = (uint32_t)s.batch.size();
+    f.stats.batch_id = s.batch_id;
+    sv->finished[s.batch[i].ticket] = f;
+    sv->inflight.erase(s.batch[i].ticket);
+  }
+  sv->bytes_in_flight -= s.batch.size() * request_state_bytes();
+  s.batch.clear();
+  s.busy = false;
+  s.launched = false;
+  sv->cv_done.notify_all();
+  sv->cv_sched.notify_all();
+}
+
+void slot_waiter(q1_server* sv, size_t idx) {
+  Slot& s = sv->slots[idx];
+  (void)hipSetDevice(sv->ctx->device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(sv->m);
+      sv->cv_slot.wait(lk, [&] { return (sv->stop && !s.busy) || (s.busy && s.launched); });
+      if (!s.busy) return;
+    }
+    const q1_status st = hipEventSynchronize(s.done) == hipSuccess ? Q1_OK : Q1_EDEVICE;
+    publish(sv, s, st);
+  }
+}
+
+void scheduler_loop(q1_server* sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  std::unique_lock<std::mutex> lk(sv->m);
+  for (;;) {
+    sv->cv_sched.wait(lk, [&] {
+      if (sv->stop) return true;
+      if (sv->queue.empty()) return false;
+      for (auto& s : sv->slots)
+        if (!s.busy) return true;
+      return false;
+    });
+    if (sv->stop && sv->queue.empty()) return;
+    if (sv->stop) {
+      // drain: fail what is still queued
+      while (!sv->queue.empty()) {
+        Finished f;
+        f.status = Q1_ECANCELED;
+        sv->finished[sv->queue.front().ticket] = f;
+        sv->queue.pop_front();
+      }
+      sv->cv_done.notify_all();
+      return;
+    }
+    // Continuous batching: while other batches run, give the queue a short window to fill.
+    bool others_busy = false;
+    for (auto& s : sv->slots) others_busy = others_busy || s.busy;
+    if (others_busy && sv->queue.size() < sv->cfg.max_batch && sv->cfg.batch_window_us) {
+      sv->cv_sched.wait_for(lk, std::chrono::microseconds(sv->cfg.batch_window_us),
+                            [&] { return sv->stop || sv->queue.size() >= sv->cfg.max_batch; });
+      if (sv->stop) continue;
+    }
+    Slot* slot = nullptr;
+    for (auto& s : sv->slots)
+      if (!s.busy) slot = &s;
+    if (!slot || sv->queue.empty()) continue;
+    // Admission: requests of the front request's layout, within the memory budget.
+    const q1_table* t0 = sv->queue.front().req.table;
+    slot->single = !t0->fast;
+    const uint32_t limit = slot->single ? 1 : sv->cfg.max_batch;
+    std::deque<Pending> rest;
+    while (!sv->queue.empty() && slot->batch.size() < limit) {
+      Pending p = sv->queue.front();
+      sv->queue.pop_front();
+      const bool same = p.req.table->layout == t0->layout && p.req.table->fast == t0->fast;
+      const bool fits = sv->bytes_in_flight + request_state_bytes() <= sv->cfg.memory_budget;
+      if (!same || !fits) {
+        rest.push_back(p);
+        if (!fits) break;
+        continue;
+      }
+      sv->bytes_in_flight += request_state_bytes();
+      sv->inflight[p.ticket] = true;
+      slot->batch.push_back(p);
+    }
+    while (!rest.empty()) {
+      sv->queue.push_front(rest.back());
+      rest.pop_back();
+    }
+    if (slot->batch.empty()) {
+      // nothing admissible until in-flight memory frees
+      sv->cv_sched.wait(lk);
+      continue;
+    }
+    slot->busy = true;
+    slot->started = Clock::now();
+    slot->batch_id = ++sv->batches;
+    lk.unlock();
+    const q1_status st = launch_slot(sv, *slot);
+    if (st != Q1_OK) {
+      publish(sv, *slot, st);
+      lk.lock();
+      continue;
+    }
+    lk.lock();
+    slot->launched = true;
+    sv->cv_slot.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server** out) {
+  if (!ctx || !out) return Q1_EINVAL;
+  *out = nullptr;
+  q1_server_config c{256, 2, 65536, 200, 64ull << 20};
+  if (cfg) {
+    if (cfg->max_batch) c.max_batch = cfg->max_batch;
+    if (cfg->max_inflight) c.max_inflight = cfg->max_inflight;
+    if (cfg->queue_capacity) c.queue_capacity = cfg->queue_capacity;
+    c.batch_window_us = cfg->batch_window_us;
+    if (cfg->memory_budget) c.memory_budget = cfg->memory_budget;
+  }
+  if (c.max_batch > 4096 || c.max_inflight > 8) return q1::fail(ctx, Q1_EINVAL, "max_batch <= 4096, max_inflight <= 8");
+  try {
+    q1_server* sv = new q1_server();
+    sv->ctx = ctx;
+    sv->cfg = c;
+    const q1_placement best = ctx->best[Q1_LAYOUT_COMPRESSED];
+    sv->placement = best == Q1_PLACE_PER_THREAD_SHARED_MEM ? best : Q1_PLACE_IN_REGISTERS;
+    sv->blocks = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 4;
+    sv->slots.resize(c.max_inflight);
+    (void)hipSetDevice(ctx->device);
+    for (auto& s : sv->slots) {
+      if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+        q1_server_destroy(sv);
+        return q1::fail(ctx, Q1_EDEVICE, "stream creation failed");
+      }
+    }
+    for (size_t i = 0; i < sv->slots.size(); i++) sv->slots[i].waiter = std::thread(slot_waiter, sv, i);
+    sv->scheduler = std::thread(scheduler_loop, sv);
+    *out = sv;
+    return Q1_OK;
+  } catch (const std::exception& e) {
+    return q1::fail(ctx, Q1_ENOMEM, e.what());
+  }
+}
+
+q1_status q1_server_submit(q1_server* sv, const q1_request* req, uint64_t* ticket) {
+  if (!sv || !req || !ticket || !req->table) return Q1_EINVAL;
+  const q1_table* t = req->table;
+  if (req->row_begin > t->rows || req->row_count > t->rows - req->row_begin) return Q1_EINVAL;
+  if (request_state_bytes() > sv->cfg.memory_budget) return Q1_ENOMEM;  // could never be admitted
+  std::lock_guard<std::mutex> g(sv->m);
+  if (sv->stop) return Q1_ECANCELED;
+  if (sv->queue.size() >= sv->cfg.queue_capacity) return Q1_EBUSY;
+  Pending p{sv->next_ticket++, *req, Clock::now()};
+  *ticket = p.ticket;
+  sv->queue.push_back(p);
+  sv->cv_sched.notify_all();
+  return Q1_OK;
+}
+
+q1_status q1_server_wait(q1_server* sv, uint64_t ticket, int64_t timeout_us, q1_result* out, q1_request_stats* stats) {
+  if (!sv) return Q1_EINVAL;
+  std::unique_lock<std::mutex> lk(sv->m);
+  auto ready = [&] { return sv->finished.count(ticket) != 0; };
+  bool known = ready() || sv->inflight.count(ticket);
+  if (!known)
+    for (auto& p : sv->queue) known = known || p.ticket == ticket;
+  if (!known) return Q1_EINVAL;
+  if (timeout_us < 0) {
+    sv->cv_done.wait(lk, ready);
+  } else if (!sv->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), ready)) {
+    return Q1_ETIMEOUT;
+  }
+  Finished f = sv->finished[ticket];
+  sv->finished.erase(ticket);
+  if (out) *out = f.result;
+  if (stats) *stats = f.stats;
+  return f.status;
+}
+
+q1_status q1_server_cancel(q1_server* sv, uint64_t ticket) {
+  if (!sv) return Q1_EINVAL;
+  std::lock_guard<std::mutex> g(sv->m);
+  for (auto it = sv->queue.begin(); it != sv->queue.end(); ++it) {
+    if (it->ticket == ticket) {
+      sv->queue.erase(it);
+      Finished f;
+      f.status = Q1_ECANCELED;
+      sv->finished[ticket] = f;
+      sv->cv_done.notify_all();
+      return Q1_OK;
+    }
+  }
+  // on the device or already finished: too late to cancel; otherwise unknown
+  return (sv->inflight.count(ticket) || sv->finished.count(ticket)) ? Q1_EBUSY : Q1_EINVAL;
+}
+
+q1_status q1_server_drain(q1_server* sv) {
+  if (!sv) return Q1_EINVAL;
+  std::unique_lock<std::mutex> lk(sv->m);
+  sv->cv_done.wait(lk, [&] {
+    if (!sv->queue.empty()) return false;
+    for (auto& s : sv->slots)
+      if (s.busy) return false;
+    return true;
+  });
+  return Q1_OK;
+}
+
+void q1_server_destroy(q1_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    sv->stop = true;
+    sv->cv_sched.notify_all();
+    sv->cv_slot.notify_all();
+  }
+  if (sv->scheduler.joinable()) sv->scheduler.join();
+  // let in-flight batches finish before the waiters exit
+  for (;;) {
+    bool busy = false;
+    {
+      std::lock_guard<std::mutex> g(sv->m);
+      for (auto& s : sv->slots) busy = busy || s.busy;
+      sv->cv_slot.notify_all();
+    }
+    if (!busy) break;
+    std::this_thread::sleep_for(std::chrono::microseconds(200));
+  }
+  {
+    std::lock_guard<std::mutex> g(sv->m);
+    sv->cv_slot.notify_all();
+  }
+  for (auto& s : sv->slots)
+    if (s.waiter.joinable()) s.waiter.join();
+  for (auto& s : sv->slots) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    free_slot_buffers(s);
+    if (s.done) (void)hipEventDestroy(s.done);
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+  }
+  delete sv;
+}
+
+}  // extern "C"
