@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
+    ap.add_argument("--copy-probe-bytes", type=int, default=0,
+                    help="after the timed region, copy this many bytes twice with a plain device copy "
+                         "kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     return ap.parse_args()
 
 
@@ -151,7 +154,8 @@ def main():
 
     # K passes accumulate K times the single-pass table: a size-independent exactness check
     host = q1.DeviceAggregates()
-    C.memmove(C.byref(host), agg.cpu().numpy().ctypes.data, C.sizeof(host))
+    agg_host = agg.cpu().numpy()   # keep the host copy alive while it is read
+    C.memmove(C.byref(host), agg_host.ctypes.data, C.sizeof(host))
     res = q1.Result()
     q1.lib.q1_aggregates_to_result(C.byref(host), C.byref(res))
     got, one = q1.result_to_list(res), q1.result_to_list(single)
@@ -161,6 +165,15 @@ def main():
     if not ok:
         log("bench: ERROR accumulated passes differ from K x single pass")
         sys.exit(1)
+
+    if args.copy_probe_bytes:
+        src = torch.ones(args.copy_probe_bytes // 8, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy probe {args.copy_probe_bytes} bytes x2 done")
+        del src, dst
 
     algo_bytes = rows * q1.BITS_PER_ROW[layout] // 8
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9

@@ -25,7 +25,8 @@ def note(*a):
 
 def device_aggregates_to_result(t):
     host = q1.DeviceAggregates()
-    C.memmove(C.byref(host), t.cpu().numpy().ctypes.data, C.sizeof(host))
+    arr = t.cpu().numpy()   # keep the host copy alive while it is read
+    C.memmove(C.byref(host), arr.ctypes.data, C.sizeof(host))
     r = q1.Result()
     q1.lib.q1_aggregates_to_result(C.byref(host), C.byref(r))
     return q1.result_to_list(r)

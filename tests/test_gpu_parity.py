@@ -160,7 +160,7 @@ def test_compressed_out_of_domain(ctx, oracle):
 
 
 def test_int128_sums_cross_2_64_general_path(ctx, oracle):
-    rows = [(729999, 1, 9 * 10**14, 0, 0, "N", "O")] * 4000 + [(729999, 1, 9 * 10**14, 200, 3, "R", "F")] * 4000
+    rows = [(729999, 1, 9 * 10**14, 0, 0, "N", "O")] * 4000 + [(729999, 1, 9 * 10**14, 200, 0, "R", "F")] * 4000
     cols = pinning.ref_columns(rows)
     want, _ = oracle.naive(cols, 0, len(rows), 729999)
     assert want[1 << 1 | 1]["sum_charge"] > 2**64 and want[2 << 1]["sum_charge"] < -(2**64)

@@ -123,9 +123,26 @@ class RequestStats(C.Structure):
                 ("batch_size", C.c_uint32), ("batch_id", C.c_uint32)]
 
 
+def _share_torch_hip_runtime():
+    """Makes this process use one HIP runtime.  torch bundles its own libamdhip64 (same
+    soname as the system one); if the system copy were loaded first, torch would load a
+    second runtime and find no GPU.  Loading torch's copy first lets libq1hip.so bind to it.
+    Q1_HIP_RUNTIME=system keeps the system runtime (for processes that never use torch)."""
+    if os.environ.get("Q1_HIP_RUNTIME", "") == "system":
+        return
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    _share_torch_hip_runtime()
     lib = C.CDLL(LIB_PATH)
     P, U64, I32, I = C.c_void_p, C.c_uint64, C.c_int32, C.c_int
     sig = {
