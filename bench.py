@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
+    ap.add_argument("--batch-requests", type=int, default=256, help="configs[3]: 0 skips the batch leg")
+    ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="configs[4]: 0 skips the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the timed region, copy this many bytes twice with a plain device copy "
                          "kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -86,6 +90,100 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
             "sample": f"{rows} rows, reference layout (38 B/row), oracle KernelNaive restatement, "
                       f"morsel-parallel, mean of {len(hot)} hot passes",
             "ms_per_pass": sec * 1e3}
+
+
+def stream_probe(torch, nbytes: int = 4 << 30):
+    """Practical HBM ceilings on this box: a device-wide reduction (read only) and a copy
+    (read + write) over nbytes with torch's kernels, timed with HIP events (best of 5)."""
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def best(fn):
+        t = []
+        for _ in range(5):
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1) * 1e-3)
+        return min(t)
+    read = best(lambda: x.sum())
+    copy = best(lambda: y.copy_(x))
+    del x, y
+    return {"bytes": nbytes, "read_gbps": nbytes / read / 1e9, "copy_gbps": 2 * nbytes / copy / 1e9,
+            "method": "torch float32 sum (read) and copy_ (read+write), best of 5"}
+
+
+def batch_leg(q1, ctx, n: int):
+    """n requests, each on its own seeded column slice (log-uniform 1e4..2e7
+    rows, cutoff uniform in 1998-06-01..1998-11-30), in one q1_run_batch (persistent
+    kernel replayed from a hipGraph) vs the plain per-request q1_run one at a time."""
+    import numpy as np
+    rng = np.random.default_rng(0x51C00003)
+    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e7), n)).astype(np.int64)
+    cutoffs = rng.integers(729906, 730089, n)
+    tables = [ctx.generate(q1.COMPRESSED, 0x51C00003 ^ r, int(sizes[r])) for r in range(n)]
+    reqs = (q1.Request * n)(*[q1.Request(tables[r].handle, 0, int(sizes[r]), int(cutoffs[r]), 0) for r in range(n)])
+    outs = (q1.Result * n)()
+    for _ in range(2):   # build the graph, then warm
+        ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
+    batched = (time.perf_counter() - t0) / reps
+    batched_dev = ctx.last_timing().kernel_ms * 1e-3
+    singles = []
+    for r in range(n):   # warm the per-request path
+        ctx.run_raw(tables[r], 0, int(sizes[r]), int(cutoffs[r]), q1.AUTO)
+    t0 = time.perf_counter()
+    for r in range(n):
+        singles.append(ctx.run_raw(tables[r], 0, int(sizes[r]), int(cutoffs[r]), q1.AUTO))
+    one = time.perf_counter() - t0
+    exact = all(q1.result_to_list(outs[r]) == q1.result_to_list(singles[r]) for r in range(n))
+    for t in tables:
+        t.free()
+    rows = int(sizes.sum())
+    return {"requests": n, "rows": rows, "batched_ms": batched * 1e3, "batched_device_ms": batched_dev * 1e3,
+            "batched_req_s": n / batched, "batched_rows_s": rows / batched,
+            "one_at_a_time_ms": one * 1e3, "one_at_a_time_req_s": n / one,
+            "speedup": one / batched, "exact_vs_single": exact}
+
+
+def serving_leg(q1, ctx, seconds: float, rate: float):
+    """Poisson arrivals over 8 HBM-resident compressed column sets
+    (1M..600M rows), random row range and cutoff per request, through the
+    continuous-batching server vs one-at-a-time dispatch of the plain kernel."""
+    import numpy as np
+
+    from tpchq01_gpu_amd.serving import Server, make_request_stream, run_open_loop
+    sizes = np.geomspace(1e6, 6e8, 8).astype(np.int64)
+    tables = [ctx.generate(q1.COMPRESSED, 0x51C00004 + k, int(sizes[k])) for k in range(8)]
+    stream = make_request_stream(tables, rate, seconds, seed=9)
+    with Server(ctx, max_batch=256, max_inflight=2, batch_window_us=200, memory_budget=64 << 20) as sv:
+        done, wall = run_open_loop(sv, stream)
+    lat = np.array([d[1] for d in done]) / 1e3
+    bs = np.array([d[2] for d in done])
+    ok = all(d[0] == q1.OK for d in done)
+    # one at a time: the same arrivals served by q1_run in order on one thread
+    t0 = time.perf_counter()
+    one_lat = []
+    for at, tab, b, nrows, c in stream:
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        ctx.run_raw(tab, b, nrows, c, q1.AUTO)
+        one_lat.append((time.perf_counter() - t0 - at) * 1e3)
+    one_wall = time.perf_counter() - t0
+    one_lat = np.array(one_lat)
+    for t in tables:
+        t.free()
+    return {"rate": rate, "seconds": seconds, "requests": len(stream), "ok": ok,
+            "req_s": len(stream) / wall, "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(bs.mean()),
+            "one_at_a_time": {"req_s": len(stream) / one_wall, "p50_ms": float(np.percentile(one_lat, 50)),
+                              "p99_ms": float(np.percentile(one_lat, 99))}}
 
 
 def main():
@@ -175,6 +273,8 @@ def main():
         log(f"bench: copy probe {args.copy_probe_bytes} bytes x2 done")
         del src, dst
 
+    stream = stream_probe(torch) if args.stream_probe else None
+
     algo_bytes = rows * q1.BITS_PER_ROW[layout] // 8
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = traffic_from_profile(args.traffic_file) if layout == q1.COMPRESSED else None
@@ -199,8 +299,18 @@ def main():
                    "bytes_per_row": q1.BITS_PER_ROW[layout] / 8},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kernel_ms, "algorithmic_bytes": algo_bytes},
+                     "kernel_ms": kernel_ms, "algorithmic_bytes": algo_bytes,
+                     "stream_probe": stream,
+                     "frac_of_stream_read": (achieved / stream["read_gbps"]) if stream else None},
     }
+    if args.batch_requests:
+        log(f"bench: batch leg, {args.batch_requests} requests")
+        out["batch"] = batch_leg(q1, ctx, args.batch_requests)
+        log(f"bench: batch {out['batch']}")
+    if args.serve_seconds > 0:
+        log(f"bench: serving leg, {args.serve_rate} req/s for {args.serve_seconds}s")
+        out["serving"] = serving_leg(q1, ctx, args.serve_seconds, args.serve_rate)
+        log(f"bench: serving {out['serving']}")
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_rows} rows x {args.cpu_passes} passes, {args.cpu_threads} threads")
         out["cpu_baseline"] = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)

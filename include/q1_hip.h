@@ -117,7 +117,7 @@ typedef struct {
 /* Launch shape (the reference's --threads-per-block / --tuples-per-thread,
  * src/main.cu:255-360, 627-650).  Zero fields mean "from the autotuning table". */
 typedef struct {
-  uint32_t threads_per_block; /* multiple of 64 */
+  uint32_t threads_per_block; /* 64, 128 or 256 (the batched kernel: 256) */
   uint32_t tuples_per_thread; /* multiple of 16 */
   uint32_t num_blocks;        /* 0 = ceil(rows / (threads_per_block * tuples_per_thread)) */
   uint32_t flags;             /* Q1_LAUNCH_* */

@@ -87,7 +87,9 @@ def test_errors(ctx):
     with pytest.raises(q1.Q1Error):
         ctx.run(t, 900, 200)                      # range past the end
     with pytest.raises(q1.Q1Error):
-        ctx.run(t, 0, 1000, launch=q1.make_launch(100, 1024))   # not a multiple of 64
+        ctx.run(t, 0, 1000, launch=q1.make_launch(100, 1024))   # not 64, 128 or 256
+    with pytest.raises(q1.Q1Error):
+        ctx.run(t, 0, 1000, launch=q1.make_launch(192, 1024))
     with pytest.raises(q1.Q1Error):
         ctx.run(t, 0, 1000, launch=q1.make_launch(256, 1000))   # not a multiple of 16
     with pytest.raises(q1.Q1Error):

@@ -76,7 +76,7 @@ def test_row_ranges(ctx, oracle, layout):
             assert as_dicts(ctx.run(t, begin, end - begin, cutoff, p)) == want
 
 
-@pytest.mark.parametrize("shape", [(64, 16), (128, 256), (192, 48), (256, 4096), (256, 16384), (64, 1024)])
+@pytest.mark.parametrize("shape", [(64, 16), (128, 256), (128, 48), (256, 4096), (256, 16384), (64, 1024)])
 def test_launch_shapes(ctx, oracle, shape):
     rows = 300_001
     ref = host_tables(rows)[q1.REFERENCE]

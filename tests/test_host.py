@@ -146,3 +146,28 @@ def test_result_merge_is_exact():
     r = q1.result_to_list(out)
     assert r[0].sum_charge == (2**64 + 2**64 - 5) + (-3 * 2**64 + 10)
     assert r[1].sum_disc_price == 1 + (-(2**64) + 2**64 - 1)
+
+
+def test_shipped_tuning_table_is_well_formed():
+    """The autotuning table that q1_init loads next to the library: every launch shape is
+    one the kernels accept and every layout names a best placement."""
+    import os
+
+    from tpchq01_gpu_amd import PACKAGE_DIR
+    path = os.path.join(PACKAGE_DIR, "tuning", "gfx950.txt")
+    launches, best = {}, {}
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "launch":
+            l, p, tpb, tpt = map(int, f[1:5])
+            assert tpb in (64, 128, 256) and tpt % 16 == 0 and 16 <= tpt <= 16384, line
+            launches[(l, p)] = (tpb, tpt)
+        else:
+            assert f[0] == "best", line
+            best[int(f[1])] = int(f[2])
+    layouts = (q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED, q1.COMPRESSED_PUSHDOWN)
+    assert set(best) == set(layouts)
+    for l in layouts:
+        assert (l, best[l]) in launches

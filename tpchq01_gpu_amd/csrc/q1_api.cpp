@@ -131,7 +131,9 @@ static void load_tuning_file(q1_ctx* ctx) {
     if (kind == "launch") {
       uint32_t tpb = 0, tpt = 0;
       ls >> l >> p >> tpb >> tpt;
-      if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL && tpb && tpt) {
+      // same limits as make_shape: a bad line is ignored rather than failing every launch
+      const bool shape_ok = (tpb == 64 || tpb == 128 || tpb == 256) && tpt && tpt % 16 == 0 && tpt <= 16384;
+      if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL && shape_ok) {
         ctx->tuning[l][p].threads_per_block = tpb;
         ctx->tuning[l][p].tuples_per_thread = tpt;
       }

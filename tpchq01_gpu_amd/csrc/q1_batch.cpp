@@ -121,8 +121,10 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     std::lock_guard<std::mutex> g(ctx->lock);
     q1_placement p = ctx->best[layout];
     if (p != Q1_PLACE_PER_THREAD_SHARED_MEM) p = Q1_PLACE_IN_REGISTERS;
-    const uint32_t threads = launch && launch->threads_per_block ? launch->threads_per_block : 256;
-    if (threads % 64 || threads > 256) return fail(ctx, Q1_EINVAL, "threads_per_block must be 64..256, multiple of 64");
+    // The batched kernel runs 256-thread blocks; only the grid size is tunable.
+    const uint32_t threads = 256;
+    if (launch && launch->threads_per_block && launch->threads_per_block != threads)
+      return fail(ctx, Q1_EINVAL, "the batched kernel runs 256 threads per block");
     const uint32_t blocks = launch && launch->num_blocks ? launch->num_blocks
                                                          : (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 4;
     uint32_t cap = 16;

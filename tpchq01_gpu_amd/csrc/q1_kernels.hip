@@ -33,6 +33,7 @@ namespace q1 {
 namespace {
 
 constexpr int kMaxBlock = 256;
+constexpr int kBatchBlock = 256;  // the batched kernel's block size
 constexpr uint32_t kNoGroup = 7;
 
 // ------------------------------------------------------------------------------------
@@ -495,41 +496,39 @@ struct AccLocal {
 };
 
 // Q1_PLACE_PER_THREAD_SHARED_MEM: each lane owns a table in LDS laid out
-// [field][group][thread] (consecutive lanes -> consecutive 8-byte words, so a wavefront's
+// [group][field][thread] (consecutive lanes -> consecutive 8-byte words, so a wavefront's
 // update of any mix of groups is bank-conflict free); rows are added with return-less
-// LDS atomics, which need no VALU add and no read-back.
+// LDS atomics, which need no VALU add and no read-back.  With the block size known at
+// compile time a row's address is one shift-add and the four fields are immediate offsets.
+template <int B>
 struct AccShared {
-  uint64_t* tab;  // dynamic LDS: 4 * 6 * blockDim words
-  uint32_t stride;
+  uint64_t* tab;  // dynamic LDS: 6 * 4 * B words
   __device__ __forceinline__ void init(char* lds) {
     tab = reinterpret_cast<uint64_t*>(lds);
-    stride = blockDim.x;
-    for (int i = 0; i < 4 * kNumGroups; i++) tab[i * stride + threadIdx.x] = 0;
+#pragma unroll
+    for (int i = 0; i < 4 * kNumGroups; i++) tab[i * B + threadIdx.x] = 0;
   }
   __device__ __forceinline__ void add(const FastRow& r) {
     if (r.counts()) {
-      uint64_t* base = tab + r.gid * stride + threadIdx.x;
-      const uint32_t f = kNumGroups * stride;
+      uint64_t* base = tab + r.gid * (4 * B) + threadIdx.x;
       __hip_atomic_fetch_add(base, pack_small64(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(base + f, (uint64_t)r.price, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(base + 2 * f, (uint64_t)disc_prod(r), __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(base + 3 * f, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + B, (uint64_t)r.price, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + 2 * B, (uint64_t)disc_prod(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + 3 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
   __device__ __forceinline__ void flush() {}
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
-    const uint32_t f = kNumGroups * stride;
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      const uint64_t* base = tab + g * stride + threadIdx.x;
+      const uint64_t* base = tab + g * (4 * B) + threadIdx.x;
       unpack_small64(base[0], o.v[g]);
-      o.v[g][F_PRICE] = base[f];
-      o.v[g][F_DPROD] = base[2 * f];
-      o.v[g][F_CHARGE] = base[3 * f];
+      o.v[g][F_PRICE] = base[B];
+      o.v[g][F_DPROD] = base[2 * B];
+      o.v[g][F_CHARGE] = base[3 * B];
     }
   }
-  static constexpr uint32_t lds_bytes(uint32_t threads) { return 4u * kNumGroups * 8u * threads; }
+  static constexpr uint32_t lds_bytes(uint32_t) { return 4u * kNumGroups * 8u * B; }
 };
 
 // Q1_PLACE_GLOBAL: one table per block in global memory (the kernel's workspace),
@@ -577,12 +576,12 @@ struct AccGlobal {
   static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
 };
 
-template <int P> struct AccOf;
-template <> struct AccOf<Q1_PLACE_IN_REGISTERS> { using A = AccRegisters; };
-template <> struct AccOf<Q1_PLACE_IN_REGISTERS_PER_THREAD> { using A = AccPerThread; };
-template <> struct AccOf<Q1_PLACE_LOCAL_MEM> { using A = AccLocal; };
-template <> struct AccOf<Q1_PLACE_PER_THREAD_SHARED_MEM> { using A = AccShared; };
-template <> struct AccOf<Q1_PLACE_GLOBAL> { using A = AccGlobal; };
+template <int P, int B> struct AccOf;
+template <int B> struct AccOf<Q1_PLACE_IN_REGISTERS, B> { using A = AccRegisters; };
+template <int B> struct AccOf<Q1_PLACE_IN_REGISTERS_PER_THREAD, B> { using A = AccPerThread; };
+template <int B> struct AccOf<Q1_PLACE_LOCAL_MEM, B> { using A = AccLocal; };
+template <int B> struct AccOf<Q1_PLACE_PER_THREAD_SHARED_MEM, B> { using A = AccShared<B>; };
+template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
 
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
@@ -649,11 +648,11 @@ __device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggreg
 // ------------------------------------------------------------------------------------
 // The fast scan kernel: layout L, placement P.
 // ------------------------------------------------------------------------------------
-template <int L, int P>
-__global__ void __launch_bounds__(kMaxBlock)
+template <int L, int P, int B>
+__global__ void __launch_bounds__(B)
 scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   using Tile = typename TileOf<L>::T;
-  using Acc = typename AccOf<P>::A;
+  using Acc = typename AccOf<P, B>::A;
   constexpr int TILE = Tile::TILE;
   constexpr int FLUSH = 16 / TILE;  // tiles between narrow-accumulator flushes
   extern __shared__ uint64_t lds[];
@@ -661,14 +660,14 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
   const uint64_t first_tile = a.row_begin / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
-  const uint64_t base = first_tile + (uint64_t)blockIdx.x * blockDim.x * tiles_per_thread;
+  const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
 
   Acc acc;
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
   acc.init(reinterpret_cast<char*>(lds));
 
   for (uint32_t k = 0; k < tiles_per_thread; k++) {
-    const uint64_t t = base + (uint64_t)k * blockDim.x + threadIdx.x;
+    const uint64_t t = base + (uint64_t)k * B + threadIdx.x;
     if (t < end_tile) {
       Tile tile;
       const bool edge = t * TILE < a.row_begin || (t + 1) * TILE > a.row_end;
@@ -694,7 +693,7 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   LaneTotals tot;
   acc.totals(tot, reinterpret_cast<char*>(lds));
   // The reduction scratch follows the placement's own LDS table.
-  uint64_t* red = lds + Acc::lds_bytes(blockDim.x) / 8;
+  uint64_t* red = lds + Acc::lds_bytes(B) / 8;
   __syncthreads();
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
@@ -939,7 +938,7 @@ template <int L, int P>
 __global__ void __launch_bounds__(kMaxBlock)
 batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
   using Tile = typename TileOf<L>::T;
-  using Acc = typename AccOf<P>::A;
+  using Acc = typename AccOf<P, kBatchBlock>::A;
   constexpr int TILE = Tile::TILE;
   constexpr int FLUSH = 16 / TILE;
   constexpr uint64_t kMaxTilesPerLane = 16384 / TILE;  // keeps lane partials exact
@@ -1018,21 +1017,26 @@ batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
 using FastFn = void (*)(ScanArgs, uint32_t, uint64_t*);
 using GenFn = void (*)(ScanArgs, uint32_t);
 
-template <int L>
-constexpr FastFn fast_fn(int p) {
-  return p == Q1_PLACE_IN_REGISTERS ? scan_fast<L, Q1_PLACE_IN_REGISTERS>
-       : p == Q1_PLACE_IN_REGISTERS_PER_THREAD ? scan_fast<L, Q1_PLACE_IN_REGISTERS_PER_THREAD>
-       : p == Q1_PLACE_LOCAL_MEM ? scan_fast<L, Q1_PLACE_LOCAL_MEM>
-       : p == Q1_PLACE_PER_THREAD_SHARED_MEM ? scan_fast<L, Q1_PLACE_PER_THREAD_SHARED_MEM>
-       : scan_fast<L, Q1_PLACE_GLOBAL>;
+template <int L, int B>
+constexpr FastFn fast_fn_b(int p) {
+  return p == Q1_PLACE_IN_REGISTERS ? scan_fast<L, Q1_PLACE_IN_REGISTERS, B>
+       : p == Q1_PLACE_IN_REGISTERS_PER_THREAD ? scan_fast<L, Q1_PLACE_IN_REGISTERS_PER_THREAD, B>
+       : p == Q1_PLACE_LOCAL_MEM ? scan_fast<L, Q1_PLACE_LOCAL_MEM, B>
+       : p == Q1_PLACE_PER_THREAD_SHARED_MEM ? scan_fast<L, Q1_PLACE_PER_THREAD_SHARED_MEM, B>
+       : scan_fast<L, Q1_PLACE_GLOBAL, B>;
 }
 
-FastFn fast_kernel(q1_layout l, q1_placement p) {
+template <int L>
+FastFn fast_fn(int p, uint32_t threads) {
+  return threads == 64 ? fast_fn_b<L, 64>(p) : threads == 128 ? fast_fn_b<L, 128>(p) : fast_fn_b<L, 256>(p);
+}
+
+FastFn fast_kernel(q1_layout l, q1_placement p, uint32_t threads) {
   switch (l) {
-    case Q1_LAYOUT_REFERENCE: return fast_fn<Q1_LAYOUT_REFERENCE>(p);
-    case Q1_LAYOUT_UNCOMPRESSED: return fast_fn<Q1_LAYOUT_UNCOMPRESSED>(p);
-    case Q1_LAYOUT_COMPRESSED: return fast_fn<Q1_LAYOUT_COMPRESSED>(p);
-    default: return fast_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(p);
+    case Q1_LAYOUT_REFERENCE: return fast_fn<Q1_LAYOUT_REFERENCE>(p, threads);
+    case Q1_LAYOUT_UNCOMPRESSED: return fast_fn<Q1_LAYOUT_UNCOMPRESSED>(p, threads);
+    case Q1_LAYOUT_COMPRESSED: return fast_fn<Q1_LAYOUT_COMPRESSED>(p, threads);
+    default: return fast_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(p, threads);
   }
 }
 
@@ -1068,8 +1072,8 @@ q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* 
   const uint32_t trows = tile_rows(layout);
   uint32_t threads = launch && launch->threads_per_block ? launch->threads_per_block : 256;
   uint32_t tpt = launch && launch->tuples_per_thread ? launch->tuples_per_thread : 1024;
-  if (threads % 64 != 0 || threads == 0 || threads > (uint32_t)kMaxBlock) {
-    *why = "threads_per_block must be a multiple of 64 in [64, 256]";
+  if (threads != 64 && threads != 128 && threads != 256) {
+    *why = "threads_per_block must be 64, 128 or 256";
     return Q1_EINVAL;
   }
   if (tpt % 16 != 0 || tpt == 0 || tpt > 16384) {
@@ -1113,10 +1117,10 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
   const hipStream_t s = (hipStream_t)stream;
   const uint32_t nwaves = shape.threads / 64;
   if (fast) {
-    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared::lds_bytes(shape.threads) : 0;
+    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? 4u * kNumGroups * 8u * shape.threads : 0;
     lds += nwaves * kNumGroups * NF * 8;
-    hipLaunchKernelGGL(fast_kernel(layout, placement), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
-                       shape.tiles_per_thread, (uint64_t*)workspace);
+    hipLaunchKernelGGL(fast_kernel(layout, placement, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds,
+                       s, a, shape.tiles_per_thread, (uint64_t*)workspace);
   } else {
     const uint32_t lds = nwaves * kNumGroups * 8 * 8;
     hipLaunchKernelGGL(general_kernel(layout), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
@@ -1185,7 +1189,8 @@ static void* batch_fn(q1_placement p) {
 q1_status launch_batch(q1_layout layout, q1_placement placement, const BatchHeader* buf,
                        q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream) {
   if (placement != Q1_PLACE_PER_THREAD_SHARED_MEM) placement = Q1_PLACE_IN_REGISTERS;
-  uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared::lds_bytes(threads) : 0;
+  if (threads != (uint32_t)kBatchBlock) return Q1_EINVAL;
+  uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared<kBatchBlock>::lds_bytes(threads) : 0;
   lds += threads / 64 * kNumGroups * NF * 8;
   void* fn;
   switch (layout) {

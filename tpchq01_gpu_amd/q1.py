@@ -16,7 +16,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libq1hip.so")
+# Q1_LIB_PATH selects another build of the library (e.g. an experimental variant).
+LIB_PATH = os.environ.get("Q1_LIB_PATH") or os.path.join(_HERE, "lib", "libq1hip.so")
 
 NUM_GROUPS = 6
 
