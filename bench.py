@@ -2,7 +2,7 @@
 """TPC-H Q1 scan -> filter -> group-aggregate on one MI355X.
 
 A step is one pass of Q1 over the whole device-resident table (600M rows, compressed
-layout: 9.375 B/row, BASELINE config 2), launched through the C ABI's q1_enqueue (the
+layout: 9.375 B/row), launched through the C ABI's q1_enqueue (the
 drop-in kernel launch) into one device aggregate table.  Timed region: K back-to-back
 passes between two synchronisations; value = rows * K / seconds.  The kernel's average
 duration comes from HIP events recorded on the library's stream around the K launches.
@@ -41,9 +41,9 @@ def parse():
     ap.add_argument("--cpu-passes", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
-    ap.add_argument("--batch-requests", type=int, default=256, help="configs[3]: 0 skips the batch leg")
+    ap.add_argument("--batch-requests", type=int, default=256, help="requests in the batch leg (0 skips it)")
     ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="configs[4]: 0 skips the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the timed region, copy this many bytes twice with a plain device copy "
@@ -72,7 +72,7 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
     (cpu/kernel_naive.hpp) on a bounded sample of the same generator's rows in the
     reference layout; the first pass is cold and dropped (cpu/q1.cpp:268-299)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from checker import Oracle
+    from oracle_lib import Oracle
 
     o = Oracle()
     with q1.Context(0) as c:
@@ -92,31 +92,21 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
             "ms_per_pass": sec * 1e3}
 
 
-def stream_probe(torch, nbytes: int = 4 << 30):
-    """Practical HBM ceilings on this box: a device-wide reduction (read only) and a copy
-    (read + write) over nbytes with torch's kernels, timed with HIP events (best of 5)."""
-    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    y = torch.empty_like(x)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-
-    def best(fn):
-        t = []
-        for _ in range(5):
-            e0.record()
-            fn()
-            e1.record()
-            torch.cuda.synchronize()
-            t.append(e0.elapsed_time(e1) * 1e-3)
-        return min(t)
-    read = best(lambda: x.sum())
-    copy = best(lambda: y.copy_(x))
-    del x, y
-    return {"bytes": nbytes, "read_gbps": nbytes / read / 1e9, "copy_gbps": 2 * nbytes / copy / 1e9,
-            "method": "torch float32 sum (read) and copy_ (read+write), best of 5"}
+def hbm_probe(rows: int):
+    """Practical HBM read ceiling: tpchq01_gpu_amd/bin/hbm_probe reads the same nine
+    compressed columns (and the same bytes as one flat buffer) with no arithmetic, in a
+    child process started before this one touches the GPU."""
+    import subprocess
+    exe = os.path.join(ROOT, "tpchq01_gpu_amd", "bin", "hbm_probe")
+    r = subprocess.run([exe, str(rows)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log(f"bench: hbm_probe failed ({r.returncode}): {r.stderr.strip()}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def batch_leg(q1, ctx, n: int):
-    """n requests, each on its own seeded column slice (log-uniform 1e4..2e7
+    """Batch leg: n requests, each on its own seeded column slice (log-uniform 1e4..2e7
     rows, cutoff uniform in 1998-06-01..1998-11-30), in one q1_run_batch (persistent
     kernel replayed from a hipGraph) vs the plain per-request q1_run one at a time."""
     import numpy as np
@@ -152,7 +142,7 @@ def batch_leg(q1, ctx, n: int):
 
 
 def serving_leg(q1, ctx, seconds: float, rate: float):
-    """Poisson arrivals over 8 HBM-resident compressed column sets
+    """Serving leg: Poisson arrivals over 8 HBM-resident compressed column sets
     (1M..600M rows), random row range and cutoff per request, through the
     continuous-batching server vs one-at-a-time dispatch of the plain kernel."""
     import numpy as np
@@ -192,6 +182,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); this build has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
+    probe = hbm_probe(args.rows) if args.stream_probe else None
+    if probe:
+        log(f"bench: read ceiling {probe['best_columns_gbps']:.0f} GB/s (columns), "
+            f"{probe['best_flat_gbps']:.0f} GB/s (flat)")
     import torch
 
     from tpchq01_gpu_amd import q1
@@ -273,11 +267,11 @@ def main():
         log(f"bench: copy probe {args.copy_probe_bytes} bytes x2 done")
         del src, dst
 
-    stream = stream_probe(torch) if args.stream_probe else None
-
     algo_bytes = rows * q1.BITS_PER_ROW[layout] // 8
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = traffic_from_profile(args.traffic_file) if layout == q1.COMPRESSED else None
+    # the probe reads the compressed columns with the scan's pattern; other layouts get its flat read
+    ceiling = (probe["best_columns_gbps"] if layout == q1.COMPRESSED else probe["best_flat_gbps"]) if probe else None
     out = {
         "metric": METRIC,
         "value": rows * args.steps / elapsed,
@@ -290,7 +284,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
-        "data": "synthetic (seeded generator, BASELINE config distributions)",
+        "data": "synthetic (seeded counter-based generator, dbgen value distributions)",
         "config": {"workload": f"TPC-H Q1, {rows} device-resident rows, {q1.LAYOUT_NAMES[layout]} layout",
                    "rows": rows, "layout": q1.LAYOUT_NAMES[layout],
                    "placement": q1.PLACEMENT_NAMES[placement],
@@ -300,8 +294,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kernel_ms, "algorithmic_bytes": algo_bytes,
-                     "stream_probe": stream,
-                     "frac_of_stream_read": (achieved / stream["read_gbps"]) if stream else None},
+                     "read_ceiling_gbps": ceiling,
+                     "frac_of_read_ceiling": achieved / ceiling if ceiling else None},
     }
     if args.batch_requests:
         log(f"bench: batch leg, {args.batch_requests} requests")

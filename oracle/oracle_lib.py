@@ -1,7 +1,7 @@
 """TEST INFRASTRUCTURE ONLY: ctypes loader of the CPU oracle (oracle/q1_oracle.c).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use this module,
-and only as the checker / the timed CPU baseline -- never as the product path.
+and only as the test reference / the timed CPU baseline -- never as the product path.
 """
 from __future__ import annotations
 

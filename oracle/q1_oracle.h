@@ -1,7 +1,7 @@
 /*
  * q1_oracle.h -- TEST INFRASTRUCTURE ONLY.
  *
- * CPU restatement of the reference's Q1 algorithm, used as the checker for the HIP path.
+ * CPU restatement of the reference's Q1 algorithm, the test reference for the HIP path.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
  * product never links, imports or calls anything here.
  *

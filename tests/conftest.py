@@ -16,7 +16,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
-    from checker import Oracle, build
+    from oracle_lib import Oracle, build
     return Oracle(build())
 
 

@@ -2,7 +2,7 @@
 import pytest
 
 import pinning
-from checker import Oracle, build
+from oracle_lib import Oracle, build
 
 # Planted mistakes in oracle/q1_oracle.c (-DQ1O_MUTANT=k) and the check meant to catch each.
 MUTANTS = {

@@ -1,0 +1,170 @@
+// hbm_probe: the practical HBM read ceiling for the Q1 scan on this GPU.
+//
+// Reads the nine compressed Q1 columns of N rows (u16 ship date, u8 quantity, u32 price,
+// u8 discount, u8 tax, 2-bit return flags, 1-bit line statuses: 9.375 bytes per row) with
+// the scan kernel's own access pattern and launch shapes, but with no filter and no
+// aggregation (each lane only XORs what it loads), and then the same number of bytes as
+// one flat buffer.  The fastest of these is the rate the scan could reach if its
+// arithmetic were free; the scan's own rate is judged against it and against the
+// published 8 TB/s.
+//
+//   hbm_probe [rows]     (default 600000000; prints one JSON line)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#define CHECK(x)                                                                  \
+  do {                                                                            \
+    const hipError_t e_ = (x);                                                    \
+    if (e_ != hipSuccess) {                                                       \
+      std::fprintf(stderr, "hbm_probe: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+      std::exit(1);                                                               \
+    }                                                                             \
+  } while (0)
+
+namespace {
+
+struct Columns {
+  const uint4* ship;      // 2 x uint4 per 16-row tile
+  const uint4* quantity;  // 1 x uint4 per tile
+  const uint4* price;     // 4 x uint4 per tile
+  const uint4* discount;
+  const uint4* tax;
+  const uint32_t* return_flag;  // 1 word per tile
+  const uint16_t* line_status;  // 1 half-word per tile
+};
+
+constexpr uint32_t kSentinel = 0x9E3779B9u;  // a lane stores only if its XOR hits this
+
+__device__ __forceinline__ uint32_t fold(uint4 v) { return v.x ^ v.y ^ v.z ^ v.w; }
+
+// The scan kernel's pattern: in iteration k thread i reads lane tile base + k*B + i.
+template <int B>
+__global__ __launch_bounds__(B) void read_columns(Columns c, uint64_t tiles, uint32_t tiles_per_thread,
+                                                  uint32_t* sink) {
+  const uint64_t begin = (uint64_t)blockIdx.x * B * tiles_per_thread;
+  const uint64_t end = begin + (uint64_t)B * tiles_per_thread < tiles ? begin + (uint64_t)B * tiles_per_thread : tiles;
+  uint32_t acc = 0;
+  for (uint64_t t = begin + threadIdx.x; t < end; t += B) {
+    const uint4 s0 = c.ship[2 * t], s1 = c.ship[2 * t + 1];
+    const uint4 q = c.quantity[t], d = c.discount[t], x = c.tax[t];
+    const uint4 p0 = c.price[4 * t], p1 = c.price[4 * t + 1], p2 = c.price[4 * t + 2], p3 = c.price[4 * t + 3];
+    acc += fold(s0) ^ fold(s1) ^ fold(q) ^ fold(d) ^ fold(x) ^ fold(p0) ^ fold(p1) ^ fold(p2) ^ fold(p3) ^
+           c.return_flag[t] ^ c.line_status[t];
+  }
+  if (acc == kSentinel) sink[0] = acc;
+}
+
+// One flat buffer, 16-byte loads, each block a contiguous range.
+template <int B>
+__global__ __launch_bounds__(B) void read_flat(const uint4* a, uint64_t n, uint32_t per_thread, uint32_t* sink) {
+  const uint64_t begin = (uint64_t)blockIdx.x * B * per_thread;
+  const uint64_t end = begin + (uint64_t)B * per_thread < n ? begin + (uint64_t)B * per_thread : n;
+  uint32_t acc = 0;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += B) acc += fold(a[i]);
+  if (acc == kSentinel) sink[0] = acc;
+}
+
+template <typename Launch>
+double median_ms(Launch launch, int reps) {
+  hipEvent_t e0, e1;
+  CHECK(hipEventCreate(&e0));
+  CHECK(hipEventCreate(&e1));
+  launch();
+  CHECK(hipGetLastError());
+  CHECK(hipDeviceSynchronize());
+  std::vector<float> t;
+  for (int r = 0; r < reps; r++) {
+    CHECK(hipEventRecord(e0));
+    launch();
+    CHECK(hipEventRecord(e1));
+    CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    CHECK(hipEventElapsedTime(&ms, e0, e1));
+    t.push_back(ms);
+  }
+  CHECK(hipEventDestroy(e0));
+  CHECK(hipEventDestroy(e1));
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const uint64_t rows = argc > 1 ? std::strtoull(argv[1], nullptr, 10) / 16 * 16 : 600000000ull;
+  if (rows < 16 * 4096) {
+    std::fprintf(stderr, "hbm_probe: at least 65536 rows\n");
+    return 2;
+  }
+  const uint64_t tiles = rows / 16;
+  const uint64_t sizes[7] = {rows * 2, rows, rows * 4, rows, rows, rows / 4, rows / 8};
+  uint64_t offs[7], total = 0;
+  for (int i = 0; i < 7; i++) {
+    offs[i] = total;
+    total += (sizes[i] + 255) / 256 * 256;
+  }
+  char* base = nullptr;
+  uint32_t* sink = nullptr;
+  CHECK(hipMalloc(&base, total));
+  CHECK(hipMalloc(&sink, 4));
+  CHECK(hipMemset(base, 0x5A, total));
+  Columns c;
+  c.ship = reinterpret_cast<const uint4*>(base + offs[0]);
+  c.quantity = reinterpret_cast<const uint4*>(base + offs[1]);
+  c.price = reinterpret_cast<const uint4*>(base + offs[2]);
+  c.discount = reinterpret_cast<const uint4*>(base + offs[3]);
+  c.tax = reinterpret_cast<const uint4*>(base + offs[4]);
+  c.return_flag = reinterpret_cast<const uint32_t*>(base + offs[5]);
+  c.line_status = reinterpret_cast<const uint16_t*>(base + offs[6]);
+  const uint64_t bytes = rows * 75 / 8;  // what the column reads touch
+  const int reps = 15;
+
+  std::printf("{\"rows\": %llu, \"bytes\": %llu, \"columns\": [", (unsigned long long)rows,
+              (unsigned long long)bytes);
+  double best_cols = 1e30, best_flat = 1e30;
+  const int shapes[5][2] = {{64, 1024}, {64, 4096}, {128, 2048}, {256, 1024}, {256, 2048}};
+  for (int s = 0; s < 5; s++) {
+    const int b = shapes[s][0];
+    const uint32_t tpt = shapes[s][1] / 16;  // tiles per thread
+    const uint64_t blocks = (tiles + (uint64_t)b * tpt - 1) / ((uint64_t)b * tpt);
+    double ms;
+    if (b == 64)
+      ms = median_ms([&] { read_columns<64><<<dim3((uint32_t)blocks), dim3(64)>>>(c, tiles, tpt, sink); }, reps);
+    else if (b == 128)
+      ms = median_ms([&] { read_columns<128><<<dim3((uint32_t)blocks), dim3(128)>>>(c, tiles, tpt, sink); }, reps);
+    else
+      ms = median_ms([&] { read_columns<256><<<dim3((uint32_t)blocks), dim3(256)>>>(c, tiles, tpt, sink); }, reps);
+    best_cols = std::min(best_cols, ms);
+    std::printf("%s{\"threads_per_block\": %d, \"tuples_per_thread\": %d, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "",
+                b, shapes[s][1], ms, bytes / ms / 1e6);
+  }
+  std::printf("], \"flat\": [");
+  const uint64_t n16 = bytes / 16;
+  const int flat[4][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 64}};
+  for (int s = 0; s < 4; s++) {
+    const int b = flat[s][0];
+    const uint32_t pt = flat[s][1];
+    const uint64_t blocks = (n16 + (uint64_t)b * pt - 1) / ((uint64_t)b * pt);
+    const uint4* a = reinterpret_cast<const uint4*>(base);
+    double ms;
+    if (b == 64)
+      ms = median_ms([&] { read_flat<64><<<dim3((uint32_t)blocks), dim3(64)>>>(a, n16, pt, sink); }, reps);
+    else if (b == 256)
+      ms = median_ms([&] { read_flat<256><<<dim3((uint32_t)blocks), dim3(256)>>>(a, n16, pt, sink); }, reps);
+    else
+      ms = median_ms([&] { read_flat<512><<<dim3((uint32_t)blocks), dim3(512)>>>(a, n16, pt, sink); }, reps);
+    best_flat = std::min(best_flat, ms);
+    std::printf("%s{\"threads_per_block\": %d, \"per_thread\": %u, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "", b, pt,
+                ms, n16 * 16 / ms / 1e6);
+  }
+  std::printf("], \"best_columns_gbps\": %.1f, \"best_flat_gbps\": %.1f}\n", bytes / best_cols / 1e6,
+              n16 * 16 / best_flat / 1e6);
+  CHECK(hipFree(base));
+  CHECK(hipFree(sink));
+  return 0;
+}

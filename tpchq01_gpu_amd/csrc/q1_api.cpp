@@ -46,12 +46,22 @@ q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p) {
   return p == Q1_PLACE_AUTO ? ctx->best[layout] : p;
 }
 
-void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, const q1_launch* in, q1_launch* out) {
+void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows, const q1_launch* in,
+                    q1_launch* out) {
   const q1_launch& t = ctx->tuning[layout][p];
   out->threads_per_block = in && in->threads_per_block ? in->threads_per_block : t.threads_per_block;
   out->tuples_per_thread = in && in->tuples_per_thread ? in->tuples_per_thread : t.tuples_per_thread;
   out->num_blocks = in ? in->num_blocks : 0;
   out->flags = in ? in->flags : 0;
+  // The tuned tuples_per_thread is measured on large tables; a smaller range halves it
+  // until the grid has at least 4 blocks per compute unit, so small requests still
+  // spread over the whole chip.  A caller-chosen value is kept as given.
+  if (!(in && in->tuples_per_thread) && !out->num_blocks) {
+    const uint64_t want = 4ull * (uint64_t)(ctx->compute_units > 0 ? ctx->compute_units : 256);
+    const uint64_t tpb = out->threads_per_block ? out->threads_per_block : 256;
+    while (out->tuples_per_thread > 16 && rows < want * tpb * out->tuples_per_thread)
+      out->tuples_per_thread = std::max<uint32_t>(16, (out->tuples_per_thread / 2) & ~15u);
+  }
 }
 
 q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes) {
@@ -510,7 +520,7 @@ static q1_status run_locked(q1_ctx* ctx, const q1_table* t, uint64_t row_begin, 
     return fail(ctx, Q1_EINVAL, "row range outside the table");
   const q1_placement p = resolve_placement(ctx, t->layout, placement);
   q1_launch l;
-  resolve_launch(ctx, t->layout, p, launch, &l);
+  resolve_launch(ctx, t->layout, p, row_count, launch, &l);
   LaunchShape shape;
   const char* why = "";
   q1_status s = make_shape(p, t->layout, &l, row_begin, row_begin + row_count, &shape, &why);
@@ -580,7 +590,7 @@ q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout, cons
   }
   const q1_placement p = resolve_placement(ctx, layout, placement);
   q1_launch l;
-  resolve_launch(ctx, layout, p, launch, &l);
+  resolve_launch(ctx, layout, p, num_tuples, launch, &l);
   LaunchShape shape;
   const char* why = "";
   q1_status s = make_shape(p, layout, &l, row_begin, row_begin + num_tuples, &shape, &why);

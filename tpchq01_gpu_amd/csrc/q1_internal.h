@@ -46,7 +46,8 @@ namespace q1 {
 q1_status fail(q1_ctx* ctx, q1_status s, const std::string& why);
 q1_status hip_fail(q1_ctx* ctx, hipError_t e, const char* what);
 q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p);
-void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, const q1_launch* in, q1_launch* out);
+void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows, const q1_launch* in,
+                    q1_launch* out);
 uint64_t algorithmic_bytes(q1_layout layout, uint64_t rows);
 q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
 void batch_destroy(q1_batch_state* b);

@@ -116,7 +116,7 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
     std::lock_guard<std::mutex> g(ctx->lock);
     const q1_placement p = resolve_placement(ctx, layout, placement);
     q1_launch l;
-    resolve_launch(ctx, layout, p, launch, &l);
+    resolve_launch(ctx, layout, p, chunk_rows, launch, &l);
     StreamState* st = nullptr;
     q1_status s = prepare(ctx, layout, chunk_rows, num_streams, &st);
     if (s != Q1_OK) return s;
@@ -227,7 +227,7 @@ q1_status q1_run_split(q1_ctx* ctx, const q1_table* t, const q1_columns* host, u
     std::lock_guard<std::mutex> g(ctx->lock);
     const q1_placement p = resolve_placement(ctx, t->layout, placement);
     q1_launch l;
-    resolve_launch(ctx, t->layout, p, launch, &l);
+    resolve_launch(ctx, t->layout, p, split, launch, &l);
     LaunchShape shape;
     const char* why = "";
     q1_status s = make_shape(p, t->layout, &l, 0, split, &shape, &why);
