@@ -61,6 +61,36 @@ def test_enqueue_composes_by_addition(ctx, oracle):
         assert as_dicts(device_aggregates_to_result(agg)) == want, p
 
 
+def test_enqueue_other_stream_and_context_stream_agree(ctx, oracle):
+    """Launches on the context stream merge large grids through the context's spread
+    tables; launches on another stream merge directly.  Both add into the caller's
+    table, and back-to-back launches on each keep adding exactly."""
+    import torch
+    rows = 3_000_017
+    ref = q1.generate_host(q1.REFERENCE, SEED + 2, rows)
+    comp = q1.generate_host(q1.COMPRESSED, SEED + 2, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    cols = t.device_columns()
+    other = torch.cuda.Stream()
+    for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM):
+        for stream in (None, other.cuda_stream):
+            agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+            torch.cuda.synchronize()
+            l = q1.make_launch(64, 16, 0, q1.LAUNCH_ASSUME_TPCH_DOMAIN)   # 2930 blocks
+            for _ in range(3):
+                ctx.check(q1.lib.q1_enqueue(ctx.handle, p, q1.COMPRESSED, C.byref(l), C.c_void_p(agg.data_ptr()),
+                                            C.byref(cols), 0, rows, 729999, C.c_void_p(stream)), "q1_enqueue")
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            got = device_aggregates_to_result(agg)
+            for g, w in zip(got, want):
+                assert (g.count, g.sum_qty, g.sum_charge, g.sum_disc_price, g.sum_base_price, g.sum_disc) == (
+                    3 * w["count"], 3 * w["sum_qty"], 3 * w["sum_charge"], 3 * w["sum_disc_price"],
+                    3 * w["sum_base_price"], 3 * w["sum_disc"]), (p, stream)
+    t.free()
+
+
 def test_wrap_torch_columns(ctx, oracle):
     import torch
     rows = 123_457

@@ -90,6 +90,43 @@ def test_launch_shapes(ctx, oracle, shape):
         t.free()
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_spread_merge_large_grids(ctx, oracle, layout):
+    """Grids of 1024 blocks or more merge through the context's spread tables and a fold
+    kernel; launches one after another must each see clean tables."""
+    rows = 2_000_003
+    ref = host_tables(rows)[q1.REFERENCE]
+    cutoff = 729906
+    want, _ = oracle.naive(ref, 0, rows, cutoff)
+    t = device_table(ctx, layout, rows, cutoff)
+    tile = {q1.REFERENCE: 4, q1.UNCOMPRESSED: 8}.get(layout, 16)
+    for p in PLACEMENTS:
+        for tpb, tpt in ((64, 16), (256, 16), (64, 32), (128, 64)):
+            blocks = -(-rows // (tpb * tpt))
+            note(f"case layout={q1.LAYOUT_NAMES[layout]} placement={p} shape={tpb}x{tpt} blocks={blocks}")
+            for rep in range(2):
+                got = ctx.run(t, 0, rows, cutoff, p, q1.make_launch(tpb, tpt))
+                assert as_dicts(got) == want, (p, tpb, tpt, rep)
+        # an offset range whose first and last tiles are partial
+        b, e = 3 * tile + 1, rows - 5
+        want_r, _ = oracle.naive(ref, b, e, cutoff)
+        assert as_dicts(ctx.run(t, b, e - b, cutoff, p, q1.make_launch(64, 16))) == want_r
+    t.free()
+
+
+def test_spread_merge_threshold(ctx, oracle):
+    """1023 blocks merge directly, 1024 through the spread tables: same answers."""
+    for rows in (1023 * 1024, 1023 * 1024 + 1, 1024 * 1024):
+        ref = q1.generate_host(q1.REFERENCE, SEED + 3, rows)
+        comp = q1.generate_host(q1.COMPRESSED, SEED + 3, rows)
+        want, _ = oracle.naive(ref, 0, rows, 729999)
+        t = ctx.upload(q1.COMPRESSED, comp, rows)
+        for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM, q1.GLOBAL):
+            note(f"case rows={rows} placement={p}")
+            assert as_dicts(ctx.run(t, 0, rows, 729999, p, q1.make_launch(64, 16))) == want
+        t.free()
+
+
 def test_cutoff_extremes(ctx, oracle):
     rows = 50_000
     ref = host_tables(rows)[q1.REFERENCE]
@@ -185,8 +222,10 @@ def test_int128_carry_fast_path(ctx):
     t = ctx.upload(q1.COMPRESSED, cols, n)
     charge = n * (100 - disc) * price * (100 + tax)
     assert charge > 2**64
-    for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM, q1.IN_REGISTERS_PER_THREAD):
-        g = ctx.run(t, 0, n, 729999, p)[1 << 1 | 1]
+    # the tuned shape merges directly; 64 x 64 gives 24415 blocks and the spread merge
+    for p, launch in ((q1.IN_REGISTERS, None), (q1.PER_THREAD_SHARED_MEM, None), (q1.IN_REGISTERS_PER_THREAD, None),
+                      (q1.IN_REGISTERS, q1.make_launch(64, 64)), (q1.PER_THREAD_SHARED_MEM, q1.make_launch(64, 64))):
+        g = ctx.run(t, 0, n, 729999, p, launch)[1 << 1 | 1]
         assert (g.count, g.sum_qty, g.sum_base_price, g.sum_disc) == (n, n * qty * 100, n * price, 0)
         assert g.sum_disc_price == n * (100 - disc) * price
         assert g.sum_charge == charge

@@ -4,9 +4,9 @@
 // u8 discount, u8 tax, 2-bit return flags, 1-bit line statuses: 9.375 bytes per row) with
 // the scan kernel's own access pattern and launch shapes, but with no filter and no
 // aggregation (each lane only XORs what it loads), and then the same number of bytes as
-// one flat buffer.  The fastest of these is the rate the scan could reach if its
-// arithmetic were free; the scan's own rate is judged against it and against the
-// published 8 TB/s.
+// one flat buffer.  Each shape is timed as 20 back-to-back launches, as bench.py times
+// the scan.  The fastest of these is the rate the scan could reach if its arithmetic
+// were free; the scan's own rate is judged against it and against the published 8 TB/s.
 //
 //   hbm_probe [rows]     (default 600000000; prints one JSON line)
 #include <hip/hip_runtime.h>
@@ -15,7 +15,6 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
-#include <vector>
 
 #define CHECK(x)                                                                  \
   do {                                                                            \
@@ -69,28 +68,27 @@ __global__ __launch_bounds__(B) void read_flat(const uint4* a, uint64_t n, uint3
   if (acc == kSentinel) sink[0] = acc;
 }
 
+// Average launch time over `reps` back-to-back launches between two events (after two
+// warm-up launches): the same method bench.py uses for the scan, so sustained clocks
+// and HBM state are alike.
 template <typename Launch>
-double median_ms(Launch launch, int reps) {
+double back_to_back_ms(Launch launch, int reps) {
   hipEvent_t e0, e1;
   CHECK(hipEventCreate(&e0));
   CHECK(hipEventCreate(&e1));
   launch();
+  launch();
   CHECK(hipGetLastError());
   CHECK(hipDeviceSynchronize());
-  std::vector<float> t;
-  for (int r = 0; r < reps; r++) {
-    CHECK(hipEventRecord(e0));
-    launch();
-    CHECK(hipEventRecord(e1));
-    CHECK(hipEventSynchronize(e1));
-    float ms = 0;
-    CHECK(hipEventElapsedTime(&ms, e0, e1));
-    t.push_back(ms);
-  }
+  CHECK(hipEventRecord(e0));
+  for (int r = 0; r < reps; r++) launch();
+  CHECK(hipEventRecord(e1));
+  CHECK(hipEventSynchronize(e1));
+  float ms = 0;
+  CHECK(hipEventElapsedTime(&ms, e0, e1));
   CHECK(hipEventDestroy(e0));
   CHECK(hipEventDestroy(e1));
-  std::sort(t.begin(), t.end());
-  return t[t.size() / 2];
+  return ms / reps;
 }
 
 }  // namespace
@@ -122,23 +120,23 @@ int main(int argc, char** argv) {
   c.return_flag = reinterpret_cast<const uint32_t*>(base + offs[5]);
   c.line_status = reinterpret_cast<const uint16_t*>(base + offs[6]);
   const uint64_t bytes = rows * 75 / 8;  // what the column reads touch
-  const int reps = 15;
+  const int reps = 20;
 
   std::printf("{\"rows\": %llu, \"bytes\": %llu, \"columns\": [", (unsigned long long)rows,
               (unsigned long long)bytes);
   double best_cols = 1e30, best_flat = 1e30;
-  const int shapes[5][2] = {{64, 1024}, {64, 4096}, {128, 2048}, {256, 1024}, {256, 2048}};
-  for (int s = 0; s < 5; s++) {
+  const int shapes[7][2] = {{64, 1024}, {64, 4096}, {128, 2048}, {256, 256}, {256, 512}, {256, 1024}, {256, 2048}};
+  for (int s = 0; s < 7; s++) {
     const int b = shapes[s][0];
     const uint32_t tpt = shapes[s][1] / 16;  // tiles per thread
     const uint64_t blocks = (tiles + (uint64_t)b * tpt - 1) / ((uint64_t)b * tpt);
     double ms;
     if (b == 64)
-      ms = median_ms([&] { read_columns<64><<<dim3((uint32_t)blocks), dim3(64)>>>(c, tiles, tpt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_columns<64><<<dim3((uint32_t)blocks), dim3(64)>>>(c, tiles, tpt, sink); }, reps);
     else if (b == 128)
-      ms = median_ms([&] { read_columns<128><<<dim3((uint32_t)blocks), dim3(128)>>>(c, tiles, tpt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_columns<128><<<dim3((uint32_t)blocks), dim3(128)>>>(c, tiles, tpt, sink); }, reps);
     else
-      ms = median_ms([&] { read_columns<256><<<dim3((uint32_t)blocks), dim3(256)>>>(c, tiles, tpt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_columns<256><<<dim3((uint32_t)blocks), dim3(256)>>>(c, tiles, tpt, sink); }, reps);
     best_cols = std::min(best_cols, ms);
     std::printf("%s{\"threads_per_block\": %d, \"tuples_per_thread\": %d, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "",
                 b, shapes[s][1], ms, bytes / ms / 1e6);
@@ -153,11 +151,11 @@ int main(int argc, char** argv) {
     const uint4* a = reinterpret_cast<const uint4*>(base);
     double ms;
     if (b == 64)
-      ms = median_ms([&] { read_flat<64><<<dim3((uint32_t)blocks), dim3(64)>>>(a, n16, pt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_flat<64><<<dim3((uint32_t)blocks), dim3(64)>>>(a, n16, pt, sink); }, reps);
     else if (b == 256)
-      ms = median_ms([&] { read_flat<256><<<dim3((uint32_t)blocks), dim3(256)>>>(a, n16, pt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_flat<256><<<dim3((uint32_t)blocks), dim3(256)>>>(a, n16, pt, sink); }, reps);
     else
-      ms = median_ms([&] { read_flat<512><<<dim3((uint32_t)blocks), dim3(512)>>>(a, n16, pt, sink); }, reps);
+      ms = back_to_back_ms([&] { read_flat<512><<<dim3((uint32_t)blocks), dim3(512)>>>(a, n16, pt, sink); }, reps);
     best_flat = std::min(best_flat, ms);
     std::printf("%s{\"threads_per_block\": %d, \"per_thread\": %u, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "", b, pt,
                 ms, n16 * 16 / ms / 1e6);

@@ -77,6 +77,22 @@ q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes) {
   return Q1_OK;
 }
 
+q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
+                  const LaunchShape& shape, void* stream) {
+  // Launches on the context stream run one after another, so they can share one set of
+  // spread tables; launches on other streams merge directly.
+  const bool own = stream == (void*)ctx->stream && ctx->spread.tables;
+  if (own && ctx->spread_dirty) {
+    const q1_status z = launch_zero(ctx->spread.tables, kSpreadTables, stream);
+    if (z != Q1_OK) return z;
+    ctx->spread_dirty = false;
+  }
+  const q1_status s = launch_scan(placement, layout, fast, a, shape, ctx->workspace, stream,
+                                  own ? &ctx->spread : nullptr);
+  if (s != Q1_OK && own) ctx->spread_dirty = true;
+  return s;
+}
+
 // Column sizes in bytes, in q1_columns order.
 static void column_bytes(q1_layout layout, uint64_t n, uint64_t b[8]) {
   std::memset(b, 0, 8 * sizeof(uint64_t));
@@ -209,6 +225,9 @@ q1_status q1_init(int device, q1_ctx** out) {
   ok = ok && hipMalloc(&ctx->d_aggr, sizeof(q1_device_aggregates)) == hipSuccess;
   ok = ok && hipHostMalloc(&ctx->h_aggr, sizeof(q1_device_aggregates), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipMalloc(&ctx->d_stats, 8 * sizeof(int64_t)) == hipSuccess;
+  ok = ok && hipMalloc(&ctx->spread.tables, kSpreadTables * sizeof(q1_device_aggregates)) == hipSuccess;
+  ok = ok && hipMemset(ctx->spread.tables, 0, kSpreadTables * sizeof(q1_device_aggregates)) == hipSuccess;
+  ok = ok && hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     q1_shutdown(ctx);
     return Q1_EDEVICE;
@@ -226,6 +245,7 @@ void q1_shutdown(q1_ctx* ctx) {
   stream_state_release(ctx);
   if (ctx->workspace) (void)hipFree(ctx->workspace);
   if (ctx->d_stats) (void)hipFree(ctx->d_stats);
+  if (ctx->spread.tables) (void)hipFree(ctx->spread.tables);
   if (ctx->h_aggr) (void)hipHostFree(ctx->h_aggr);
   if (ctx->d_aggr) (void)hipFree(ctx->d_aggr);
   for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_h2d_begin, ctx->ev_h2d_end})
@@ -537,7 +557,7 @@ static q1_status run_locked(q1_ctx* ctx, const q1_table* t, uint64_t row_begin, 
   s = launch_zero(ctx->d_aggr, 1, ctx->stream);
   if (s != Q1_OK) return fail(ctx, s, "zero kernel launch failed");
   Q1_HIP(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
-  s = launch_scan(p, t->layout, t->fast, a, shape, ctx->workspace, ctx->stream);
+  s = scan_on(ctx, p, t->layout, t->fast, a, shape, ctx->stream);
   if (s != Q1_OK) return fail(ctx, s, "scan kernel launch failed");
   Q1_HIP(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
   Q1_HIP(ctx, hipMemcpyAsync(ctx->h_aggr, ctx->d_aggr, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,
@@ -609,7 +629,7 @@ q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout, cons
   a.cutoff_day = cutoff_day;
   a.out = aggregates;
   const bool fast = (l.flags & Q1_LAUNCH_ASSUME_TPCH_DOMAIN) != 0;
-  s = launch_scan(p, layout, fast, a, shape, ctx->workspace, stream);
+  s = scan_on(ctx, p, layout, fast, a, shape, stream);
   return s == Q1_OK ? Q1_OK : fail(ctx, s, "scan kernel launch failed");
   GUARD_END(ctx)
 }

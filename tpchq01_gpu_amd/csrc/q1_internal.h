@@ -39,6 +39,8 @@ struct q1_ctx {
   q1_placement best[4] = {Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS,
                           Q1_PLACE_IN_REGISTERS};
   q1_batch_state* batch = nullptr;
+  q1::Spread spread{};                     // spread-merge tables of the context stream
+  bool spread_dirty = false;               // a failed launch may have left partials in them
   std::mutex lock;                         // serialises calls that share the context stream
 };
 
@@ -50,6 +52,9 @@ void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows
                     q1_launch* out);
 uint64_t algorithmic_bytes(q1_layout layout, uint64_t rows);
 q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
+// launch_scan with the context's spread tables when `stream` is the context stream.
+q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
+                  const LaunchShape& shape, void* stream);
 void batch_destroy(q1_batch_state* b);
 void stream_state_release(q1_ctx* ctx);
 }  // namespace q1

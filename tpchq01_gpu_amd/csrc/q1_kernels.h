@@ -26,15 +26,30 @@ struct ScanArgs {
   uint64_t row_begin, row_end;
   int32_t cutoff_day;       // MonetDB day number (ignored for pushdown)
   q1_device_aggregates* out;
+  uint32_t out_mask;        // set by launch_scan: block b merges into out[b & out_mask]
+};
+
+// Spread merge.  Thousands of blocks adding into one 48-word table serialise on those
+// words' atomics at the end of the grid; instead block b of a large launch merges into
+// tables[b % kSpreadTables] and a one-block fold kernel adds the tables into the
+// destination and clears them.  The tables must be zero before the launch (the fold
+// leaves them zero) and must not be shared by launches that can overlap: one set per
+// stream that serialises its launches.
+constexpr uint32_t kSpreadTables = 64;      // power of two
+constexpr uint32_t kSpreadMinBlocks = 1024; // smaller grids merge directly
+struct Spread {
+  q1_device_aggregates* tables;  // kSpreadTables zeroed tables
 };
 
 // Validate and complete a launch shape for `rows` rows (0 fields filled from defaults).
 q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* launch,
                      uint64_t row_begin, uint64_t row_end, LaunchShape* shape, const char** why);
 
-// Enqueue the scan kernel.  `fast` selects the narrow (TPC-H domain) kernels.
+// Enqueue the scan kernel.  `fast` selects the narrow (TPC-H domain) kernels.  With a
+// `spread` (may be null) a fast launch of at least kSpreadMinBlocks blocks merges through
+// the spread tables and is followed by the fold kernel on the same stream.
 q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
-                      const LaunchShape& shape, void* workspace, void* stream);
+                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread = nullptr);
 uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape);
 
 // Zero a device aggregate table (a fill kernel, safe inside graph capture).

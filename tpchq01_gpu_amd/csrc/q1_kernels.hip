@@ -500,6 +500,8 @@ struct AccLocal {
 // update of any mix of groups is bank-conflict free); rows are added with return-less
 // LDS atomics, which need no VALU add and no read-back.  With the block size known at
 // compile time a row's address is one shift-add and the four fields are immediate offsets.
+constexpr uint32_t shared_table_bytes(uint32_t threads) { return 4u * kNumGroups * 8u * threads; }
+
 template <int B>
 struct AccShared {
   uint64_t* tab;  // dynamic LDS: 6 * 4 * B words
@@ -528,7 +530,7 @@ struct AccShared {
       o.v[g][F_CHARGE] = base[3 * B];
     }
   }
-  static constexpr uint32_t lds_bytes(uint32_t) { return 4u * kNumGroups * 8u * B; }
+  static constexpr uint32_t lds_bytes(uint32_t) { return shared_table_bytes(B); }
 };
 
 // Q1_PLACE_GLOBAL: one table per block in global memory (the kernel's workspace),
@@ -586,10 +588,31 @@ template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
 // ------------------------------------------------------------------------------------
+// DPP move of a 64-bit value (two 32-bit lane permutes); CTRL is a gfx9 DPP control.
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+  const uint2 w = __builtin_bit_cast(uint2, v);
+  const uint2 r = make_uint2((uint32_t)__builtin_amdgcn_mov_dpp((int)w.x, CTRL, 0xF, 0xF, false),
+                             (uint32_t)__builtin_amdgcn_mov_dpp((int)w.y, CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(uint64_t, r);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  const uint2 w = __builtin_bit_cast(uint2, v);
+  const uint2 r = make_uint2((uint32_t)__builtin_amdgcn_readlane((int)w.x, lane),
+                             (uint32_t)__builtin_amdgcn_readlane((int)w.y, lane));
+  return __builtin_bit_cast(uint64_t, r);
+}
+
+// Sum of a 64-bit value over the 64 lanes of a wavefront (all lanes active); the result
+// is wavefront-uniform.  Within each row of 16 lanes: quad butterflies (quad_perm
+// [1,0,3,2] and [2,3,0,1]) then rotations by 4 and 8 leave the row total in every lane;
+// the four row totals are read into scalar registers and added there.
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
+  v += dpp_u64<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_u64<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_u64<0x124>(v);  // row_ror:4
+  v += dpp_u64<0x128>(v);  // row_ror:8
+  return readlane_u64(v, 0) + readlane_u64(v, 16) + readlane_u64(v, 32) + readlane_u64(v, 48);
 }
 
 // 128-bit signed add of (lo, hi) into a device (lo, hi) word pair with two atomics.
@@ -618,29 +641,36 @@ __device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggreg
     }
   }
   __syncthreads();
-  if (threadIdx.x < kNumGroups) {
-    const int g = threadIdx.x;
-    uint64_t lo[NF], hi[NF];
-    for (int f = 0; f < NF; f++) lo[f] = hi[f] = 0;
+  // Thread g*NF + f sums field f of group g over the waves (128-bit) and issues that
+  // field's atomics, so a block's destination atomics go out together.
+  if (threadIdx.x < kNumGroups * NF) {
+    const int g = threadIdx.x / NF, f = threadIdx.x % NF;
+    uint64_t lo = 0, hi = 0, cnt = 0, plo = 0, phi = 0;
     for (uint32_t w = 0; w < nwaves; w++) {
-      for (int f = 0; f < NF; f++) {
-        const uint64_t v = red[(w * kNumGroups + g) * NF + f];
-        lo[f] += v;
-        hi[f] += lo[f] < v;
-      }
+      const uint64_t* r = red + (w * kNumGroups + g) * NF;
+      const uint64_t v = r[f];
+      lo += v;
+      hi += lo < v;
+      cnt |= r[F_CNT];
+      const uint64_t pv = r[F_PRICE];
+      plo += pv;
+      phi += plo < pv;
     }
-    if (lo[F_CNT] | hi[F_CNT]) {
-      atomicAdd((unsigned long long*)&out->sum_qty[g], (unsigned long long)(lo[F_QTY] * qty_scale));
-      atomicAdd((unsigned long long*)&out->sum_base_price[g], (unsigned long long)lo[F_PRICE]);
-      atomicAdd((unsigned long long*)&out->sum_disc[g], (unsigned long long)lo[F_DISC]);
-      atomicAdd((unsigned long long*)&out->count[g], (unsigned long long)lo[F_CNT]);
-      // sum_disc_price = 100 * sum_price - sum(disc * price), in 128 bits
-      const unsigned __int128 p100 = ((unsigned __int128)hi[F_PRICE] << 64 | lo[F_PRICE]) * 100u;
-      const unsigned __int128 dprod = (unsigned __int128)hi[F_DPROD] << 64 | lo[F_DPROD];
-      const unsigned __int128 dp = p100 - dprod;
-      atomic_add128(&out->sum_disc_price_lo[g], &out->sum_disc_price_hi[g], (uint64_t)dp,
-                    (uint64_t)(dp >> 64));
-      atomic_add128(&out->sum_charge_lo[g], &out->sum_charge_hi[g], lo[F_CHARGE], hi[F_CHARGE]);
+    if (cnt) {
+      switch (f) {
+        case F_QTY: atomicAdd((unsigned long long*)&out->sum_qty[g], (unsigned long long)(lo * qty_scale)); break;
+        case F_PRICE: atomicAdd((unsigned long long*)&out->sum_base_price[g], (unsigned long long)lo); break;
+        case F_DISC: atomicAdd((unsigned long long*)&out->sum_disc[g], (unsigned long long)lo); break;
+        case F_CNT: atomicAdd((unsigned long long*)&out->count[g], (unsigned long long)lo); break;
+        case F_DPROD: {
+          // sum_disc_price = 100 * sum_price - sum(disc * price), in 128 bits
+          const unsigned __int128 p100 = ((unsigned __int128)phi << 64 | plo) * 100u;
+          const unsigned __int128 dp = p100 - ((unsigned __int128)hi << 64 | lo);
+          atomic_add128(&out->sum_disc_price_lo[g], &out->sum_disc_price_hi[g], (uint64_t)dp, (uint64_t)(dp >> 64));
+          break;
+        }
+        default: atomic_add128(&out->sum_charge_lo[g], &out->sum_charge_hi[g], lo, hi); break;
+      }
     }
   }
 }
@@ -697,7 +727,7 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   __syncthreads();
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-  block_merge(tot, red, a.out, qty_scale);
+  block_merge(tot, red, a.out + (blockIdx.x & a.out_mask), qty_scale);
 }
 
 // ------------------------------------------------------------------------------------
@@ -829,6 +859,54 @@ scan_general(ScanArgs a, uint32_t tiles_per_thread) {
 __global__ void zero_words(uint64_t* p, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
+}
+
+// Adds the kSpreadTables spread tables into `out` and clears them.  One block of four
+// wavefronts; lane k reads table k, so every load of the block is issued at once and the
+// sums are wavefront butterflies.  The 48 words of a table split over the waves:
+//   wave 0: sum_qty, sum_base_price       (words  0..11, 64-bit)
+//   wave 1: sum_disc_price lo/hi pairs    (words 12..23, 128-bit)
+//   wave 2: sum_charge lo/hi pairs        (words 24..35, 128-bit)
+//   wave 3: sum_disc, count               (words 36..47, 64-bit)
+// Each wave clears only the words it read, after its sums consumed them.
+static_assert(kSpreadTables == 64, "the fold gives each table one lane of a wavefront");
+static_assert(sizeof(q1_device_aggregates) == 48 * 8, "8 fields x 6 groups");
+__global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables, q1_device_aggregates* out) {
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t* mine = reinterpret_cast<uint64_t*>(tables + lane) + wave * 12;
+  uint64_t* dst = reinterpret_cast<uint64_t*>(out) + wave * 12;
+  uint64_t v[12];
+#pragma unroll
+  for (int i = 0; i < 12; i++) v[i] = mine[i];
+  // After the butterflies every lane holds every sum; lane i then adds word (or pair) i,
+  // so the wave's destination atomics are issued together rather than one after another.
+  if (wave == 0 || wave == 3) {
+    uint64_t my = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      const uint64_t sum = wave_sum_u64(v[i]);
+      my = lane == (uint32_t)i ? sum : my;
+    }
+    if (lane < 12 && my) atomicAdd((unsigned long long*)&dst[lane], (unsigned long long)my);
+  } else {
+    uint64_t my_lo = 0, my_hi = 0;
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      uint64_t lo = v[g], hi = v[6 + g];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+        const uint64_t olo = __shfl_xor(lo, m, 64), ohi = __shfl_xor(hi, m, 64);
+        const uint64_t nlo = lo + olo;
+        hi += ohi + (nlo < lo);
+        lo = nlo;
+      }
+      my_lo = lane == (uint32_t)g ? lo : my_lo;
+      my_hi = lane == (uint32_t)g ? hi : my_hi;
+    }
+    if (lane < (uint32_t)kNumGroups) atomic_add128(&dst[lane], &dst[6 + lane], my_lo, my_hi);
+  }
+#pragma unroll
+  for (int i = 0; i < 12; i++) mine[i] = 0;
 }
 
 // One thread per 32-row group writes the rows' values and the packed words of the group.
@@ -1111,16 +1189,28 @@ uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape) 
   return placement == Q1_PLACE_GLOBAL ? (uint64_t)shape.blocks * AccGlobal::W * kNumGroups * 8 : 0;
 }
 
-q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
-                      const LaunchShape& shape, void* workspace, void* stream) {
+q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& args,
+                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread) {
   if (shape.blocks == 0) return Q1_OK;
   const hipStream_t s = (hipStream_t)stream;
   const uint32_t nwaves = shape.threads / 64;
+  ScanArgs a = args;
+  a.out_mask = 0;
   if (fast) {
-    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? 4u * kNumGroups * 8u * shape.threads : 0;
+    const bool spread_merge = spread && spread->tables && shape.blocks >= kSpreadMinBlocks;
+    if (spread_merge) {
+      a.out = spread->tables;
+      a.out_mask = kSpreadTables - 1;
+    }
+    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? shared_table_bytes(shape.threads) : 0;
     lds += nwaves * kNumGroups * NF * 8;
     hipLaunchKernelGGL(fast_kernel(layout, placement, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds,
                        s, a, shape.tiles_per_thread, (uint64_t*)workspace);
+    if (spread_merge) {
+      const q1_status st = hip_status(hipGetLastError());
+      if (st != Q1_OK) return st;
+      hipLaunchKernelGGL(fold_spread, dim3(1), dim3(256), 0, s, spread->tables, args.out);
+    }
   } else {
     const uint32_t lds = nwaves * kNumGroups * 8 * 8;
     hipLaunchKernelGGL(general_kernel(layout), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
