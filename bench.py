@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
     ap.add_argument("--batch-requests", type=int, default=256, help="requests in the batch leg (0 skips it)")
     ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
+    ap.add_argument("--no-layouts", dest="layouts", action="store_false",
+                    help="skip the pushdown / uncompressed / reference layouts leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the serving leg (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
@@ -139,6 +141,35 @@ def batch_leg(q1, ctx, n: int):
             "batched_req_s": n / batched, "batched_rows_s": rows / batched,
             "one_at_a_time_ms": one * 1e3, "one_at_a_time_req_s": n / one,
             "speedup": one / batched, "exact_vs_single": exact}
+
+
+def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5):
+    """The same rows in the other layouts (pushdown bitmap built on the device from the
+    compressed table; uncompressed and reference layouts generated from the same seed):
+    kernel time of q1_run with the tuned placement, mean of `reps` runs after one warm-up,
+    and a check that every layout returns the compressed table's result exactly."""
+    want = q1.result_to_list(ctx.run_raw(comp_table, 0, rows, 729999, q1.AUTO))
+    out, exact = {}, True
+    for name, make in (("compressed_pushdown", lambda: ctx.build_filter(comp_table, 729999)),
+                       ("uncompressed", lambda: ctx.generate(q1.UNCOMPRESSED, seed, rows)),
+                       ("reference", lambda: ctx.generate(q1.REFERENCE, seed, rows))):
+        t = make()
+        got = q1.result_to_list(ctx.run_raw(t, 0, rows, 729999, q1.AUTO))
+        exact = exact and got == want
+        ms = []
+        for _ in range(reps):
+            ctx.run_raw(t, 0, rows, 729999, q1.AUTO)
+            ms.append(ctx.last_timing().kernel_ms)
+        tm = ctx.last_timing()
+        ms = sum(ms) / len(ms)
+        algo = rows * q1.BITS_PER_ROW[t.layout] // 8
+        out[name] = {"bytes_per_row": q1.BITS_PER_ROW[t.layout] / 8, "kernel_ms": ms,
+                     "gbps": algo / (ms * 1e-3) / 1e9, "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "rows_s": rows / (ms * 1e-3), "placement": q1.PLACEMENT_NAMES[tm.placement],
+                     "threads_per_block": tm.threads_per_block, "tuples_per_thread": tm.tuples_per_thread}
+        t.free()
+    out["exact_vs_compressed"] = exact
+    return out
 
 
 def serving_leg(q1, ctx, seconds: float, rate: float):
@@ -297,6 +328,10 @@ def main():
                      "read_ceiling_gbps": ceiling,
                      "frac_of_read_ceiling": achieved / ceiling if ceiling else None},
     }
+    if args.layouts and layout == q1.COMPRESSED:
+        log("bench: layouts leg")
+        out["layouts"] = layouts_leg(q1, ctx, table, rows, 0x51C00002)
+        log(f"bench: layouts {out['layouts']}")
     if args.batch_requests:
         log(f"bench: batch leg, {args.batch_requests} requests")
         out["batch"] = batch_leg(q1, ctx, args.batch_requests)

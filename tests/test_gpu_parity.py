@@ -232,6 +232,27 @@ def test_int128_carry_fast_path(ctx):
     t.free()
 
 
+def test_shared_narrow_price_bound(ctx):
+    """At 256 rows per lane the shared placement keeps a lane's price sum in 32 bits:
+    256 rows of the largest domain price fill it to 2^32 - 256.  272 rows per lane take
+    the 64-bit price field.  Closed form."""
+    price = (1 << 24) - 1
+    for tpt in (256, 272):
+        n = 64 * tpt * 5 + 7          # five full blocks of 64 lanes and a partial one
+        cols = {
+            "ship_date": np.zeros(n, np.uint16), "quantity": np.full(n, 50, np.uint8),
+            "extended_price": np.full(n, price, np.uint32), "discount": np.full(n, 10, np.uint8),
+            "tax": np.full(n, 8, np.uint8),
+            "return_flag": np.full((n + 15) // 16, 0xAAAAAAAA, np.uint32),   # every row 'R'
+            "line_status": np.zeros((n + 31) // 32, np.uint32),              # every row 'F'
+        }
+        t = ctx.upload(q1.COMPRESSED, cols, n)
+        g = ctx.run(t, 0, n, 729999, q1.PER_THREAD_SHARED_MEM, q1.make_launch(64, tpt))[2 << 1]
+        assert (g.count, g.sum_qty, g.sum_base_price, g.sum_disc) == (n, n * 5000, n * price, n * 10), tpt
+        assert g.sum_disc_price == n * 90 * price and g.sum_charge == n * 90 * 108 * price, tpt
+        t.free()
+
+
 def test_split_matches_oracle(ctx, oracle):
     rows = 1_000_003
     h = host_tables(rows)

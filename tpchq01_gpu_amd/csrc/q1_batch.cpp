@@ -126,7 +126,7 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     if (launch && launch->threads_per_block && launch->threads_per_block != threads)
       return fail(ctx, Q1_EINVAL, "the batched kernel runs 256 threads per block");
     const uint32_t blocks = launch && launch->num_blocks ? launch->num_blocks
-                                                         : (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 4;
+                                                         : (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 8;
     uint32_t cap = 16;
     while (cap < n) cap <<= 1;
     q1_status s = build_state(ctx, layout, p, cap, threads, blocks);

@@ -141,7 +141,7 @@ struct CompressedTile {
     return (int32_t)s <= cut;
   }
   __device__ __forceinline__ uint32_t gid(int j) const {
-    return ((rf >> (2 * j)) & 3u) << 1 | ((ls >> j) & 1u);
+    return __builtin_amdgcn_ubfe(rf, 2 * j, 2) << 1 | __builtin_amdgcn_ubfe(ls, j, 1);
   }
   __device__ __forceinline__ uint32_t byte(const uint4& v, int j) const {
     return (comp(v, j >> 2) >> (8 * (j & 3))) & 0xFFu;
@@ -502,7 +502,10 @@ struct AccLocal {
 // compile time a row's address is one shift-add and the four fields are immediate offsets.
 constexpr uint32_t shared_table_bytes(uint32_t threads) { return 4u * kNumGroups * 8u * threads; }
 
-template <int B>
+// NARROW: at most 256 rows per lane, so a lane's price sum stays below 256 * 2^24 = 2^32
+// and the price field takes a 32-bit LDS atomic on the low word of its 64-bit slot (no
+// register pair to build per row); the slot's high word stays zero.
+template <int B, bool NARROW = false>
 struct AccShared {
   uint64_t* tab;  // dynamic LDS: 6 * 4 * B words
   __device__ __forceinline__ void init(char* lds) {
@@ -514,7 +517,11 @@ struct AccShared {
     if (r.counts()) {
       uint64_t* base = tab + r.gid * (4 * B) + threadIdx.x;
       __hip_atomic_fetch_add(base, pack_small64(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(base + B, (uint64_t)r.price, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (NARROW)
+        __hip_atomic_fetch_add(reinterpret_cast<uint32_t*>(base + B), r.price, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      else
+        __hip_atomic_fetch_add(base + B, (uint64_t)r.price, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __hip_atomic_fetch_add(base + 2 * B, (uint64_t)disc_prod(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __hip_atomic_fetch_add(base + 3 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
@@ -584,6 +591,9 @@ template <int B> struct AccOf<Q1_PLACE_IN_REGISTERS_PER_THREAD, B> { using A = A
 template <int B> struct AccOf<Q1_PLACE_LOCAL_MEM, B> { using A = AccLocal; };
 template <int B> struct AccOf<Q1_PLACE_PER_THREAD_SHARED_MEM, B> { using A = AccShared<B>; };
 template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
+// Internal kernel variant of the shared placement for launches of at most 256 rows per lane.
+constexpr int kPlaceSharedNarrow = 8;
+template <int B> struct AccOf<kPlaceSharedNarrow, B> { using A = AccShared<B, true>; };
 
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
@@ -1101,6 +1111,7 @@ constexpr FastFn fast_fn_b(int p) {
        : p == Q1_PLACE_IN_REGISTERS_PER_THREAD ? scan_fast<L, Q1_PLACE_IN_REGISTERS_PER_THREAD, B>
        : p == Q1_PLACE_LOCAL_MEM ? scan_fast<L, Q1_PLACE_LOCAL_MEM, B>
        : p == Q1_PLACE_PER_THREAD_SHARED_MEM ? scan_fast<L, Q1_PLACE_PER_THREAD_SHARED_MEM, B>
+       : p == kPlaceSharedNarrow ? scan_fast<L, kPlaceSharedNarrow, B>
        : scan_fast<L, Q1_PLACE_GLOBAL, B>;
 }
 
@@ -1109,7 +1120,7 @@ FastFn fast_fn(int p, uint32_t threads) {
   return threads == 64 ? fast_fn_b<L, 64>(p) : threads == 128 ? fast_fn_b<L, 128>(p) : fast_fn_b<L, 256>(p);
 }
 
-FastFn fast_kernel(q1_layout l, q1_placement p, uint32_t threads) {
+FastFn fast_kernel(q1_layout l, int p, uint32_t threads) {
   switch (l) {
     case Q1_LAYOUT_REFERENCE: return fast_fn<Q1_LAYOUT_REFERENCE>(p, threads);
     case Q1_LAYOUT_UNCOMPRESSED: return fast_fn<Q1_LAYOUT_UNCOMPRESSED>(p, threads);
@@ -1204,7 +1215,10 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
     }
     uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? shared_table_bytes(shape.threads) : 0;
     lds += nwaves * kNumGroups * NF * 8;
-    hipLaunchKernelGGL(fast_kernel(layout, placement, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds,
+    const bool narrow = placement == Q1_PLACE_PER_THREAD_SHARED_MEM &&
+                        (uint64_t)shape.tiles_per_thread * tile_rows(layout) <= 256;
+    const int variant = narrow ? kPlaceSharedNarrow : (int)placement;
+    hipLaunchKernelGGL(fast_kernel(layout, variant, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds,
                        s, a, shape.tiles_per_thread, (uint64_t*)workspace);
     if (spread_merge) {
       const q1_status st = hip_status(hipGetLastError());
