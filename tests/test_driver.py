@@ -22,7 +22,8 @@ def run(args, cwd=None, timeout=300):
     (["notanoption"], "Usage"),
     (["--use-filter-pushdown"], "only currently supported when compression is applied"),
     (["--hash-table-placement=hashy"], 'No kernel variant named "hashy"'),
-    (["--threads-per-block=100"], "multiple of the wavefront size"),
+    (["--threads-per-block=100"], "64, 128 or 256"),
+    (["--threads-per-block=192"], "64, 128 or 256"),
     (["--runs=0"], "Number of runs must be positive"),
     (["--cpu-fraction=1.5"], "range 0.0 - 1.0"),
     (["--sf=0"], "Invalid scale factor"),

@@ -129,8 +129,8 @@ Params parse(int argc, char** argv, bool* want_device) {
       } else if (name == "threads-per-block") {
         p.threads_per_block = (uint32_t)std::stoul(value);
         p.user_threads = true;
-        if (p.threads_per_block % 64 != 0)
-          throw std::invalid_argument("threads per block must be a multiple of the wavefront size (64)");
+        if (p.threads_per_block != 64 && p.threads_per_block != 128 && p.threads_per_block != 256)
+          throw std::invalid_argument("threads per block must be 64, 128 or 256 (whole wavefronts of 64)");
       } else if (name == "tuples-per-kernel-launch" || name == "tuples-per-kernel") {
         p.tuples_per_launch = std::stoull(value);
       } else if (name == "runs") {
