@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
     ap.add_argument("--no-layouts", dest="layouts", action="store_false",
                     help="skip the pushdown / uncompressed / reference layouts leg")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="length of the serving leg (0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="length of each offered rate in the serving leg (0 skips it)")
+    ap.add_argument("--serve-rates", default="2000,8000", help="offered request rates (req/s), comma-separated")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the timed region, copy this many bytes twice with a plain device copy "
                          "kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -172,39 +173,45 @@ def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5):
     return out
 
 
-def serving_leg(q1, ctx, seconds: float, rate: float):
+def serving_leg(q1, ctx, seconds: float, rates):
     """Serving leg: Poisson arrivals over 8 HBM-resident compressed column sets
     (1M..600M rows), random row range and cutoff per request, through the
-    continuous-batching server vs one-at-a-time dispatch of the plain kernel."""
+    continuous-batching server vs one-at-a-time dispatch of the plain kernel, at each
+    offered rate in `rates`."""
     import numpy as np
 
     from tpchq01_gpu_amd.serving import Server, make_request_stream, run_open_loop
     sizes = np.geomspace(1e6, 6e8, 8).astype(np.int64)
     tables = [ctx.generate(q1.COMPRESSED, 0x51C00004 + k, int(sizes[k])) for k in range(8)]
-    stream = make_request_stream(tables, rate, seconds, seed=9)
-    with Server(ctx, max_batch=256, max_inflight=2, batch_window_us=200, memory_budget=64 << 20) as sv:
-        done, wall = run_open_loop(sv, stream)
-    lat = np.array([d[1] for d in done]) / 1e3
-    bs = np.array([d[2] for d in done])
-    ok = all(d[0] == q1.OK for d in done)
-    # one at a time: the same arrivals served by q1_run in order on one thread
-    t0 = time.perf_counter()
-    one_lat = []
-    for at, tab, b, nrows, c in stream:
-        now = time.perf_counter() - t0
-        if at > now:
-            time.sleep(at - now)
-        ctx.run_raw(tab, b, nrows, c, q1.AUTO)
-        one_lat.append((time.perf_counter() - t0 - at) * 1e3)
-    one_wall = time.perf_counter() - t0
-    one_lat = np.array(one_lat)
+    out = []
+    for rate in rates:
+        stream = make_request_stream(tables, rate, seconds, seed=9)
+        with Server(ctx, max_batch=256, max_inflight=2, batch_window_us=200, memory_budget=64 << 20) as sv:
+            done, wall = run_open_loop(sv, stream)
+        lat = np.array([d[1] for d in done]) / 1e3
+        bs = np.array([d[2] for d in done])
+        ok = all(d[0] == q1.OK for d in done)
+        # one at a time: the same arrivals served by q1_run in order on one thread
+        t0 = time.perf_counter()
+        one_lat = []
+        for at, tab, b, nrows, c in stream:
+            now = time.perf_counter() - t0
+            if at > now:
+                time.sleep(at - now)
+            ctx.run_raw(tab, b, nrows, c, q1.AUTO)
+            one_lat.append((time.perf_counter() - t0 - at) * 1e3)
+        one_wall = time.perf_counter() - t0
+        one_lat = np.array(one_lat)
+        out.append({"rate": rate, "seconds": seconds, "requests": len(stream), "ok": ok,
+                    "mean_rows": float(np.mean([r[3] for r in stream])),
+                    "req_s": len(stream) / wall, "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(bs.mean()),
+                    "one_at_a_time": {"req_s": len(stream) / one_wall, "p50_ms": float(np.percentile(one_lat, 50)),
+                                      "p99_ms": float(np.percentile(one_lat, 99))}})
+        log(f"bench: serving {out[-1]}")
     for t in tables:
         t.free()
-    return {"rate": rate, "seconds": seconds, "requests": len(stream), "ok": ok,
-            "req_s": len(stream) / wall, "p50_ms": float(np.percentile(lat, 50)),
-            "p99_ms": float(np.percentile(lat, 99)), "mean_batch": float(bs.mean()),
-            "one_at_a_time": {"req_s": len(stream) / one_wall, "p50_ms": float(np.percentile(one_lat, 50)),
-                              "p99_ms": float(np.percentile(one_lat, 99))}}
+    return out
 
 
 def main():
@@ -337,9 +344,9 @@ def main():
         out["batch"] = batch_leg(q1, ctx, args.batch_requests)
         log(f"bench: batch {out['batch']}")
     if args.serve_seconds > 0:
-        log(f"bench: serving leg, {args.serve_rate} req/s for {args.serve_seconds}s")
-        out["serving"] = serving_leg(q1, ctx, args.serve_seconds, args.serve_rate)
-        log(f"bench: serving {out['serving']}")
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        log(f"bench: serving leg, {rates} req/s for {args.serve_seconds}s each")
+        out["serving"] = serving_leg(q1, ctx, args.serve_seconds, rates)
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_rows} rows x {args.cpu_passes} passes, {args.cpu_threads} threads")
         out["cpu_baseline"] = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)

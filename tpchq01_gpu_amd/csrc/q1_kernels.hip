@@ -360,6 +360,7 @@ __device__ __forceinline__ uint64_t fast_charge(const FastRow& r) {
 // Q1_PLACE_IN_REGISTERS: narrow per-tile accumulators in registers; a group's block of
 // adds is skipped by the whole wavefront when no lane holds a row of that group.
 struct AccRegisters {
+  static constexpr bool kPackedSmall = false;
   uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
   uint64_t ch[kNumGroups], pr[kNumGroups], dp[kNumGroups];
   uint32_t cnt[kNumGroups], dsc[kNumGroups], qty[kNumGroups];
@@ -425,6 +426,7 @@ struct AccRegisters {
 // sums per group in registers, every row updates its group through an unrolled
 // compare-select over all six groups (src/kernels/ht_in_registers_per_thread.cuh:54-64).
 struct AccPerThread {
+  static constexpr bool kPackedSmall = false;
   uint64_t v[kNumGroups][NF];
   __device__ __forceinline__ void init(char*) {
 #pragma unroll
@@ -471,6 +473,7 @@ __device__ __forceinline__ void unpack_small64(uint64_t s, uint64_t* out) {
 // Q1_PLACE_LOCAL_MEM: the per-thread table is indexed by the row's group at run time,
 // so it lives in the lane's private (scratch) memory; slot 7 absorbs non-counting rows.
 struct AccLocal {
+  static constexpr bool kPackedSmall = false;
   volatile uint64_t t[8][4];  // [slot][small, price, dprod, charge]
   __device__ __forceinline__ void init(char*) {
     for (int g = 0; g < 8; g++)
@@ -527,11 +530,20 @@ struct AccShared {
     }
   }
   __device__ __forceinline__ void flush() {}
+  // NARROW tables hand over the packed (count | discount | quantity) word in F_CNT: at
+  // most 256 rows per lane, a wavefront's sum of it still fits every field (count
+  // < 2^17, discount < 2^20, quantity < 2^27), so the block merge sums one word for three.
+  static constexpr bool kPackedSmall = NARROW;
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       const uint64_t* base = tab + g * (4 * B) + threadIdx.x;
-      unpack_small64(base[0], o.v[g]);
+      if (NARROW) {
+        o.v[g][F_CNT] = base[0];
+        o.v[g][F_QTY] = o.v[g][F_DISC] = 0;
+      } else {
+        unpack_small64(base[0], o.v[g]);
+      }
       o.v[g][F_PRICE] = base[B];
       o.v[g][F_DPROD] = base[2 * B];
       o.v[g][F_CHARGE] = base[3 * B];
@@ -545,6 +557,7 @@ struct AccShared {
 // lanes of the block, so it sees up to blockDim * 16384 rows: count and discount share one
 // word (count bits [0,24), discount [24,64)), the other sums have a word each.
 struct AccGlobal {
+  static constexpr bool kPackedSmall = false;
   static constexpr int W = 5;  // words per group: count|disc, qty, price, dprod, charge
   uint64_t* tab;               // [W][6] words of this block
   __device__ __forceinline__ void init_global(uint64_t* ws) {
@@ -638,16 +651,31 @@ __device__ __forceinline__ void atomic_add128(uint64_t* lo_p, uint64_t* hi_p, ui
 // Block-wide reduction of LaneTotals (exact: lane sums fit uint64, block sums are taken
 // in 128 bits) and the merge into `out`.  `red` is LDS scratch of
 // (blockDim/64) * 6 * NF uint64 words.  qty_scale turns layout units into x100.
+// PACKED: t.v[g][F_CNT] holds the packed (count | discount | quantity) word of the
+// lane (pack_small64 layout) and F_QTY / F_DISC are unused.
+template <bool PACKED = false>
 __device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggregates* out,
                             uint64_t qty_scale) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
 #pragma unroll
   for (int g = 0; g < kNumGroups; g++) {
     const bool any = ballot(t.v[g][F_CNT] != 0) != 0;
+    uint64_t* r = red + (wave * kNumGroups + g) * NF;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
+      if (PACKED && (f == F_QTY || f == F_DISC)) continue;
       const uint64_t s = any ? wave_sum_u64(t.v[g][f]) : 0;
-      if (lane == 0) red[(wave * kNumGroups + g) * NF + f] = s;
+      if (lane == 0) {
+        if (PACKED && f == F_CNT) {
+          uint64_t u[NF];
+          unpack_small64(s, u);
+          r[F_CNT] = u[F_CNT];
+          r[F_DISC] = u[F_DISC];
+          r[F_QTY] = u[F_QTY];
+        } else {
+          r[f] = s;
+        }
+      }
     }
   }
   __syncthreads();
@@ -737,7 +765,7 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   __syncthreads();
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-  block_merge(tot, red, a.out + (blockIdx.x & a.out_mask), qty_scale);
+  block_merge<Acc::kPackedSmall>(tot, red, a.out + (blockIdx.x & a.out_mask), qty_scale);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1092,7 +1120,7 @@ batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
     __syncthreads();
     const uint64_t qty_scale =
         (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-    block_merge(tot, red, &outs[r], qty_scale);
+    block_merge<Acc::kPackedSmall>(tot, red, &outs[r], qty_scale);
     __syncthreads();  // red and the placement's LDS table are reused by the next segment
     pos = seg_end;
     if (pos >= req_end) r++;
