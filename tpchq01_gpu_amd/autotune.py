@@ -34,7 +34,12 @@ def sweep(ctx: q1.Context, layout: int, rows: int, placements, reps: int, seed: 
     algo = rows * q1.BITS_PER_ROW[layout] // 8
     copies = 1 if algo >= 4 * MALL_BYTES else max(2, (4 * MALL_BYTES) // max(algo, 1) + 1)
     copies = min(copies, 8)
-    tables = [ctx.generate(layout, seed + i, rows) for i in range(copies)]
+    if layout == q1.COMPRESSED_PUSHDOWN:
+        # the pushdown bitmap is built on the device from a compressed table (which the
+        # pushdown table keeps alive)
+        tables = [ctx.build_filter(ctx.generate(q1.COMPRESSED, seed + i, rows), 729999) for i in range(copies)]
+    else:
+        tables = [ctx.generate(layout, seed + i, rows) for i in range(copies)]
     cus = ctx.device_info().compute_units
     results = []
     try:
@@ -98,7 +103,8 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--out", default=os.path.join(PACKAGE_DIR, "tuning"))
     args = ap.parse_args(argv)
-    names = {"reference": q1.REFERENCE, "uncompressed": q1.UNCOMPRESSED, "compressed": q1.COMPRESSED}
+    names = {"reference": q1.REFERENCE, "uncompressed": q1.UNCOMPRESSED, "compressed": q1.COMPRESSED,
+             "compressed_pushdown": q1.COMPRESSED_PUSHDOWN}
     layouts = [names[x] for x in args.layouts.split(",")]
     placements = [q1.PLACEMENTS[x] for x in args.placements.split(",")]
     t0 = time.time()
