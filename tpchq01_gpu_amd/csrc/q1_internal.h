@@ -22,6 +22,9 @@ struct q1_table {
 };
 
 struct q1_batch_state;  // q1_batch.cpp
+namespace q1 {
+struct StreamState;     // q1_stream.cpp: the host-resident streaming pipeline
+}
 
 struct q1_ctx {
   int device = 0;
@@ -39,6 +42,7 @@ struct q1_ctx {
   q1_placement best[4] = {Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS,
                           Q1_PLACE_IN_REGISTERS};
   q1_batch_state* batch = nullptr;
+  q1::StreamState* streaming = nullptr;    // cached staging buffers and streams of q1_run_streaming
   q1::Spread spread{};                     // spread-merge tables of the context stream
   bool spread_dirty = false;               // a failed launch may have left partials in them
   std::mutex lock;                         // serialises calls that share the context stream

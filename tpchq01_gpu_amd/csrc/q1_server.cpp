@@ -322,7 +322,7 @@ q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server**
     sv->cfg = c;
     const q1_placement best = ctx->best[Q1_LAYOUT_COMPRESSED];
     sv->placement = best == Q1_PLACE_PER_THREAD_SHARED_MEM ? best : Q1_PLACE_IN_REGISTERS;
-    sv->blocks = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 4;
+    sv->blocks = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 8;  // as q1_run_batch
     sv->slots.resize(c.max_inflight);
     (void)hipSetDevice(ctx->device);
     for (auto& s : sv->slots) {

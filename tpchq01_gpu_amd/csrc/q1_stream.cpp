@@ -35,52 +35,39 @@ static void stream_state_free(StreamState* st) {
   delete st;
 }
 
-// One cached pipeline per context (released with the context).
-static std::mutex g_states_lock;
-static std::vector<std::pair<q1_ctx*, StreamState*>> g_states;
-
-static StreamState*& state_of(q1_ctx* ctx) {
-  std::lock_guard<std::mutex> g(g_states_lock);
-  for (auto& p : g_states)
-    if (p.first == ctx) return p.second;
-  g_states.emplace_back(ctx, nullptr);
-  return g_states.back().second;
-}
-
 void stream_state_release(q1_ctx* ctx) {
-  std::lock_guard<std::mutex> g(g_states_lock);
-  for (auto it = g_states.begin(); it != g_states.end(); ++it) {
-    if (it->first == ctx) {
-      stream_state_free(it->second);
-      g_states.erase(it);
-      return;
-    }
-  }
+  stream_state_free(ctx->streaming);
+  ctx->streaming = nullptr;
 }
 
 static void column_bytes_local(q1_layout layout, uint64_t n, uint64_t b[8]) {
   q1_layout_column_bytes(layout, n, b);
 }
 
+// The context's cached pipeline for (layout, chunk_rows, streams), rebuilt when those
+// change.  A pipeline is stored only once all of its streams, events and staging buffers
+// exist, so a failed build leaves nothing half-made behind.
 static q1_status prepare(q1_ctx* ctx, q1_layout layout, uint64_t chunk_rows, int streams, StreamState** out) {
-  StreamState*& st = state_of(ctx);
-  if (st && st->layout == layout && st->chunk_rows == chunk_rows && st->streams == streams) {
-    *out = st;
+  StreamState* cur = ctx->streaming;
+  if (cur && cur->layout == layout && cur->chunk_rows == chunk_rows && cur->streams == streams) {
+    *out = cur;
     return Q1_OK;
   }
-  stream_state_free(st);
-  st = new StreamState();
-  st->layout = layout;
-  st->chunk_rows = chunk_rows;
-  st->streams = streams;
+  stream_state_free(cur);
+  ctx->streaming = nullptr;
+  StreamState* st = new StreamState();
+  auto give_up = [&](q1_status code, const char* why) {
+    stream_state_free(st);
+    return fail(ctx, code, why);
+  };
   uint64_t b[8];
   column_bytes_local(layout, chunk_rows, b);
   for (int i = 0; i < streams; i++) {
     hipStream_t s;
     hipEvent_t e;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return fail(ctx, Q1_EDEVICE, "stream creation failed");
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return give_up(Q1_EDEVICE, "stream creation failed");
     st->s.push_back(s);
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail(ctx, Q1_EDEVICE, "event creation failed");
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return give_up(Q1_EDEVICE, "event creation failed");
     st->done.push_back(e);
     q1_columns c{};
     const void** slots[8] = {&c.ship_date, &c.quantity, &c.extended_price, &c.discount,
@@ -88,13 +75,17 @@ static q1_status prepare(q1_ctx* ctx, q1_layout layout, uint64_t chunk_rows, int
     for (int k = 0; k < 8; k++) {
       if (!b[k]) continue;
       void* p = nullptr;
-      if (hipMalloc(&p, b[k]) != hipSuccess) return fail(ctx, Q1_ENOMEM, "staging allocation failed");
+      if (hipMalloc(&p, b[k]) != hipSuccess) return give_up(Q1_ENOMEM, "staging allocation failed");
       st->allocs.push_back(p);
       *slots[k] = p;
     }
     st->staging.push_back(c);
     st->workspace.push_back(nullptr);
   }
+  st->layout = layout;
+  st->chunk_rows = chunk_rows;
+  st->streams = streams;
+  ctx->streaming = st;
   *out = st;
   return Q1_OK;
 }
@@ -138,11 +129,18 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
     column_bytes_local(layout, 1, elem);  // per-row bytes of the unpacked columns
     const bool packed = layout == Q1_LAYOUT_COMPRESSED || layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN;
     const auto w0 = std::chrono::steady_clock::now();
+    // Once copies are queued, an error return first drains every stream, so no copy is
+    // still reading the caller's host columns when the call returns.
+    auto drain_fail = [&](q1_status code, const char* why) {
+      for (hipStream_t ss : st->s) (void)hipStreamSynchronize(ss);
+      (void)hipStreamSynchronize(ctx->stream);
+      return fail(ctx, code, why);
+    };
     s = launch_zero(ctx->d_aggr, 1, ctx->stream);
     if (s != Q1_OK) return fail(ctx, s, "zero kernel failed");
     if (hipEventRecord(ctx->ev_begin, ctx->stream) != hipSuccess) return fail(ctx, Q1_EDEVICE, "event record failed");
     for (hipStream_t ss : st->s)
-      if (hipStreamWaitEvent(ss, ctx->ev_begin, 0) != hipSuccess) return fail(ctx, Q1_EDEVICE, "stream wait failed");
+      if (hipStreamWaitEvent(ss, ctx->ev_begin, 0) != hipSuccess) return drain_fail(Q1_EDEVICE, "stream wait failed");
     const void* hsrc[8] = {host->ship_date, host->quantity, host->extended_price, host->discount,
                            host->tax, host->return_flag, host->line_status, host->filter};
     uint64_t bytes = 0;
@@ -166,14 +164,14 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
           src_off = off * elem[k];
           len = n * elem[k];
         }
-        if (!hsrc[k]) return fail(ctx, Q1_EINVAL, "a column required by the layout is NULL");
+        if (!hsrc[k]) return drain_fail(Q1_EINVAL, "a column required by the layout is NULL");
         if (hipMemcpyAsync((void*)ddst[k], (const char*)hsrc[k] + src_off, len, hipMemcpyHostToDevice, ss) != hipSuccess)
-          return fail(ctx, Q1_EDEVICE, "chunk copy failed");
+          return drain_fail(Q1_EDEVICE, "chunk copy failed");
         bytes += len;
       }
       LaunchShape shape;
       s = make_shape(p, layout, &l, 0, n, &shape, &why);
-      if (s != Q1_OK) return fail(ctx, s, why);
+      if (s != Q1_OK) return drain_fail(s, why);
       ScanArgs a;
       a.cols = dst;
       a.row_begin = 0;
@@ -181,18 +179,18 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
       a.cutoff_day = cutoff_day;
       a.out = ctx->d_aggr;
       s = launch_scan(p, layout, fast, a, shape, st->workspace[(size_t)idx], ss);
-      if (s != Q1_OK) return fail(ctx, s, "scan kernel launch failed");
+      if (s != Q1_OK) return drain_fail(s, "scan kernel launch failed");
     }
     for (int i = 0; i < num_streams; i++) {
       if (hipEventRecord(st->done[(size_t)i], st->s[(size_t)i]) != hipSuccess ||
           hipStreamWaitEvent(ctx->stream, st->done[(size_t)i], 0) != hipSuccess)
-        return fail(ctx, Q1_EDEVICE, "stream join failed");
+        return drain_fail(Q1_EDEVICE, "stream join failed");
     }
     if (hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess ||
         hipMemcpyAsync(ctx->h_aggr, ctx->d_aggr, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost, ctx->stream) !=
             hipSuccess ||
         hipStreamSynchronize(ctx->stream) != hipSuccess)
-      return fail(ctx, Q1_EDEVICE, "final copy failed");
+      return drain_fail(Q1_EDEVICE, "final copy failed");
     q1_aggregates_to_result(ctx->h_aggr, out);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);
