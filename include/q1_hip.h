@@ -184,7 +184,9 @@ q1_status q1_table_download(q1_ctx* ctx, const q1_table* t, const q1_columns* ho
 uint64_t q1_table_device_bytes(const q1_table* t);
 void q1_table_free(q1_table* t);
 /* Pushdown producer on the device: builds a COMPRESSED_PUSHDOWN table sharing the
- * compressed table's columns plus a filter bitmap for `cutoff_day`. */
+ * compressed table's columns plus a filter bitmap for `cutoff_day`.  The pushdown table
+ * owns only its bitmap: free it before the compressed table, and use it only while the
+ * compressed table lives. */
 q1_status q1_build_filter(q1_ctx* ctx, const q1_table* compressed, int32_t cutoff_day,
                           q1_table** pushdown_out);
 

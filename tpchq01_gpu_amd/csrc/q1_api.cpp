@@ -327,11 +327,17 @@ q1_status q1_table_upload(q1_ctx* ctx, q1_layout layout, const q1_columns* host,
   q1_status s = alloc_table(ctx, layout, rows, &t);
   if (s != Q1_OK) return s;
   const auto w0 = std::chrono::steady_clock::now();
-  Q1_HIP(ctx, hipEventRecord(ctx->ev_h2d_begin, ctx->stream));
-  for (int i = 0; i < 8; i++)
-    if (b[i]) Q1_HIP(ctx, hipMemcpyAsync((void*)colc(t->dev, i), colc(*host, i), b[i], hipMemcpyHostToDevice, ctx->stream));
-  Q1_HIP(ctx, hipEventRecord(ctx->ev_h2d_end, ctx->stream));
-  Q1_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  bool ok = hipEventRecord(ctx->ev_h2d_begin, ctx->stream) == hipSuccess;
+  for (int i = 0; i < 8 && ok; i++)
+    if (b[i])
+      ok = hipMemcpyAsync((void*)colc(t->dev, i), colc(*host, i), b[i], hipMemcpyHostToDevice, ctx->stream) ==
+           hipSuccess;
+  ok = ok && hipEventRecord(ctx->ev_h2d_end, ctx->stream) == hipSuccess;
+  ok = hipStreamSynchronize(ctx->stream) == hipSuccess && ok;
+  if (!ok) {
+    q1_table_free(t);
+    return fail(ctx, Q1_EDEVICE, "upload of the table columns failed");
+  }
   float ms = 0;
   (void)hipEventElapsedTime(&ms, ctx->ev_h2d_begin, ctx->ev_h2d_end);
   ctx->last = q1_timing{};
