@@ -70,10 +70,23 @@ def traffic_from_profile(path: str):
     return None
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(q1, rows: int, threads: int, passes: int):
-    """The oracle's morsel-parallel restatement of the reference CPU kernel KernelNaive
-    (cpu/kernel_naive.hpp) on a bounded sample of the same generator's rows in the
-    reference layout; the first pass is cold and dropped (cpu/q1.cpp:268-299)."""
+    """The oracle's restatements of the reference CPU kernels on a bounded sample of the
+    same generator's rows (reference layout; ComprData for X100): KernelNaive
+    (cpu/kernel_naive.hpp) morsel-parallel on `threads` threads (the reported value) and on
+    one thread, and KernelX100 (cpu/kernel_x100.hpp, 1024-row vectors) on both.  Each is
+    the mean of the hot passes after one cold pass (cpu/q1.cpp:268-299)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_lib import Oracle
 
@@ -82,17 +95,28 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
         t = c.generate(q1.REFERENCE, 0x51C00000, rows)
         cols = t.download()
         t.free()
-    times = []
-    for _ in range(passes):
-        t0 = time.perf_counter()
-        o.naive_parallel(cols, 0, rows, 729999, threads)
-        times.append(time.perf_counter() - t0)
-    hot = times[1:] if len(times) > 1 else times
-    sec = sum(hot) / len(hot)
-    return {"value": rows / sec, "unit": "rows/s", "cores": threads, "kind": "port",
+    comp = o.compact(cols, rows)
+
+    def timed(fn):
+        times = []
+        for _ in range(passes):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        hot = times[1:] if len(times) > 1 else times
+        return rows / (sum(hot) / len(hot))
+
+    naive_mt = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, threads))
+    naive_1 = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, 1))
+    x100_mt = timed(lambda: o.x100(comp, 0, rows, 729999, threads))
+    x100_1 = timed(lambda: o.x100(comp, 0, rows, 729999, 1))
+    return {"value": naive_mt, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"{rows} rows, reference layout (38 B/row), oracle KernelNaive restatement, "
-                      f"morsel-parallel, mean of {len(hot)} hot passes",
-            "ms_per_pass": sec * 1e3}
+                      f"morsel-parallel on {threads} threads, mean of {passes - 1} hot passes",
+            "ms_per_pass": rows / naive_mt * 1e3,
+            "naive_1_thread_rows_s": naive_1,
+            "x100_rows_s": x100_mt, "x100_1_thread_rows_s": x100_1,
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def hbm_probe(rows: int):
@@ -338,6 +362,37 @@ def main():
     if args.layouts and layout == q1.COMPRESSED:
         log("bench: layouts leg")
         out["layouts"] = layouts_leg(q1, ctx, table, rows, 0x51C00002)
+        # the same columns through the 64-bit kernel that runs outside the TPC-H value domain
+        # (q1_enqueue without Q1_LAUNCH_ASSUME_TPCH_DOMAIN), same accumulate-and-compare check
+        wide = q1.make_launch(256, 1024, 0, 0)
+        with torch.cuda.stream(stream):
+            agg.zero_()
+        torch.cuda.synchronize()
+        k = 5
+        for _ in range(2):
+            ctx.check(q1.lib.q1_enqueue(ctx.handle, q1.IN_REGISTERS_PER_THREAD, layout, C.byref(wide), p_agg,
+                                        C.byref(dev_cols), 0, rows, 729999, None), "q1_enqueue")
+        ctx.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            ctx.check(q1.lib.q1_enqueue(ctx.handle, q1.IN_REGISTERS_PER_THREAD, layout, C.byref(wide), p_agg,
+                                        C.byref(dev_cols), 0, rows, 729999, None), "q1_enqueue")
+        e1.record(stream)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        wms = e0.elapsed_time(e1) / k
+        host = q1.DeviceAggregates()
+        agg_host = agg.cpu().numpy()
+        C.memmove(C.byref(host), agg_host.ctypes.data, C.sizeof(host))
+        res = q1.Result()
+        q1.lib.q1_aggregates_to_result(C.byref(host), C.byref(res))
+        exact = all(g.count == (k + 2) * s1.count and g.sum_charge == (k + 2) * s1.sum_charge
+                    for g, s1 in zip(q1.result_to_list(res), one))
+        out["layouts"]["compressed_64bit_kernel"] = {
+            "kernel_ms": wms, "gbps": algo_bytes / (wms * 1e-3) / 1e9,
+            "frac": algo_bytes / (wms * 1e-3) / 1e9 / HBM_PEAK_GBS, "threads_per_block": 256,
+            "tuples_per_thread": 1024, "exact_vs_fast": exact}
         log(f"bench: layouts {out['layouts']}")
     if args.batch_requests:
         log(f"bench: batch leg, {args.batch_requests} requests")

@@ -31,6 +31,11 @@ class OLineitem(C.Structure):
                 ["shipdate", "quantity", "extendedprice", "discount", "tax", "returnflag", "linestatus"]]
 
 
+class OCompact(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ["shipdate", "returnflag", "linestatus", "discount", "tax", "extendedprice", "quantity"]]
+
+
 class OCompressed(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in
                 ["shipdate", "quantity", "extendedprice", "discount", "tax", "returnflag", "linestatus"]]
@@ -75,6 +80,12 @@ class Oracle:
         lib.q1o_precompute_filter.argtypes = [P, I64, I32, P]
         lib.q1o_pushdown_kernel.restype = None
         lib.q1o_pushdown_kernel.argtypes = [P, C.POINTER(OCompressed), I64, I64, C.POINTER(OResult)]
+        lib.q1o_compact_init.restype = None
+        lib.q1o_compact_init.argtypes = [C.POINTER(OLineitem), I64, C.POINTER(OCompact)]
+        lib.q1o_x100.restype = None
+        lib.q1o_x100.argtypes = [C.POINTER(OCompact), I64, I64, I32, C.c_int, I64, C.POINTER(OResult)]
+        lib.q1o_avx512.restype = None
+        lib.q1o_avx512.argtypes = [C.POINTER(OLineitem), I64, I64, I32, C.POINTER(OResult)]
         self.lib = lib
 
     # ---- helpers ---------------------------------------------------------------------
@@ -178,6 +189,33 @@ class Oracle:
         r = OResult()
         self.lib.q1o_pushdown_kernel(filt.ctypes.data, C.byref(self._cc(ccols)), begin, end, C.byref(r))
         return self.to_list(r)
+
+
+    def compact(self, ref_cols: dict, n: int) -> dict:
+        """ComprData: the compact columns the X100 kernel reads."""
+        self._check_ref(ref_cols)
+        out = {"ship_date": np.zeros(n, np.int16), "return_flag": np.zeros(n, np.int8),
+               "line_status": np.zeros(n, np.int8), "discount": np.zeros(n, np.int8),
+               "tax": np.zeros(n, np.int8), "extended_price": np.zeros(n, np.int32),
+               "quantity": np.zeros(n, np.int16)}
+        c = OCompact(*[out[k].ctypes.data for k in ("ship_date", "return_flag", "line_status", "discount",
+                                                     "tax", "extended_price", "quantity")])
+        self.lib.q1o_compact_init(C.byref(self._li(ref_cols)), n, C.byref(c))
+        return out
+
+    def x100(self, compact_cols: dict, begin: int, end: int, cutoff: int, threads: int = 1,
+             morsel_rows: int = 10 * 1024):
+        c = OCompact(*[compact_cols[k].ctypes.data for k in ("ship_date", "return_flag", "line_status",
+                                                              "discount", "tax", "extended_price", "quantity")])
+        r = OResult()
+        self.lib.q1o_x100(C.byref(c), begin, end, cutoff, threads, morsel_rows, C.byref(r))
+        return self.to_list(r), int(r.unexpected_keys)
+
+    def avx512(self, ref_cols: dict, begin: int, end: int, cutoff: int):
+        self._check_ref(ref_cols)
+        r = OResult()
+        self.lib.q1o_avx512(C.byref(self._li(ref_cols)), begin, end, cutoff, C.byref(r))
+        return self.to_list(r), int(r.unexpected_keys)
 
 
 def ref_to_compressed_scale(groups: list[dict]) -> list[dict]:

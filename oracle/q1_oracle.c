@@ -11,6 +11,8 @@
  *   6 return-flag bit offset off by one           7 line-status bit index taken mod 16
  *   8 charge accumulated in 64 bits               9 quantity not narrowed by 100
  *  10 last filter word dropped                    11 sum_disc accumulates (1 - disc)
+ *  12 X100: the last partial vector of a morsel is dropped
+ *  13 AVX-512: the last of the 8 table lanes is left out of the final sum
  */
 #include "q1_oracle.h"
 
@@ -335,4 +337,187 @@ void q1o_pushdown_kernel(const uint32_t* filter, const q1o_compressed* c, int64_
   for (int64_t i = begin; i < end; i++)
     if ((filter[i / 32] >> (i % 32)) & 1u) aggregate_compressed(a, c, i); /* get_bit */
   acc_to_result(a, 0, out);
+}
+
+/* ---- ComprData (cpu/common.cpp:113-137, the -DGPU variant) ---------------------------- */
+void q1o_compact_init(const q1o_lineitem* li, int64_t n, q1o_compact* out) {
+  for (int64_t i = 0; i < n; i++) {
+    out->shipdate[i] = (int16_t)(li->shipdate[i] - 727563);
+    out->returnflag[i] = (int8_t)li->returnflag[i];
+    out->linestatus[i] = (int8_t)li->linestatus[i];
+    out->discount[i] = (int8_t)li->discount[i];
+    out->tax[i] = (int8_t)li->tax[i];
+    out->extendedprice[i] = (int32_t)li->extendedprice[i];
+    out->quantity[i] = (int16_t)li->quantity[i];
+  }
+}
+
+/* ---- KernelX100::task (cpu/kernel_x100.hpp:140-402) with the k1Step aggregation ------- */
+/* One vector of at most 1024 rows [off, off + n) of the compact columns.  The primitives'
+ * types are kept: select on int16 dates (vectorized.cpp:504-506), disc_1 and tax_1 in int8
+ * (:544-554), disc_price in int32 (:557-564), charge in int64 (:447-498), and sum_disc
+ * accumulates disc_1 (kernel_x100.hpp:320). */
+enum { kX100Vector = 1024 };
+
+static void x100_vector(const q1o_compact* c, int64_t off, int64_t n, int16_t date, acc_t* a,
+                        int64_t* unexpected) {
+  uint16_t sel[kX100Vector];
+  int num = 0;
+  for (int64_t i = 0; i < n; i++) /* select_int16_t: a[i] <= b */
+    if (c->shipdate[off + i] <= date) sel[num++] = (uint16_t)i;
+  const int8_t one = 100;                            /* Decimal64::ToValue(1, 0) */
+  for (int k = 0; k < num; k++) {
+    const int64_t i = off + sel[k];
+    const int s = slot_of((char)c->returnflag[i], (char)c->linestatus[i]);
+    if (s < 0) { (*unexpected)++; continue; }
+    const int8_t disc_1 = (int8_t)(one - c->discount[i]);                       /* map_disc_1 */
+    const int8_t tax_1 = (int8_t)(one + c->tax[i]);                             /* map_tax_1 */
+    const int32_t disc_price = (int32_t)((uint32_t)(int32_t)disc_1 * (uint32_t)c->extendedprice[i]);
+    const int64_t charge = (int64_t)disc_price * tax_1;                         /* map_charge */
+    acc_t* g = &a[s];
+    g->sum_qty = (int64_t)((uint64_t)g->sum_qty + (uint64_t)(int64_t)c->quantity[i]);
+    g->sum_base_price = (int64_t)((uint64_t)g->sum_base_price + (uint64_t)(int64_t)c->extendedprice[i]);
+    g->sum_disc_price += (i128)disc_price;
+    g->sum_charge += (i128)charge;
+    g->sum_disc = (int64_t)((uint64_t)g->sum_disc + (uint64_t)(int64_t)disc_1);
+    g->count++;
+  }
+}
+
+typedef struct {
+  const q1o_compact* c;
+  int64_t end;
+  int16_t date;
+  int64_t morsel;
+  _Atomic int64_t* next;
+  acc_t a[6];
+  int64_t unexpected;
+} x100_worker_t;
+
+static void* x100_worker(void* arg) {
+  x100_worker_t* w = (x100_worker_t*)arg;
+  memset(w->a, 0, sizeof(w->a));
+  w->unexpected = 0;
+  for (;;) {
+    const int64_t off = atomic_fetch_add(w->next, w->morsel);
+    if (off >= w->end) break;
+    const int64_t stop = off + w->morsel < w->end ? off + w->morsel : w->end;
+    for (int64_t v = off; v < stop; v += kX100Vector) {
+      const int64_t n = stop - v < kX100Vector ? stop - v : kX100Vector;
+#if Q1O_MUTANT == 12
+      if (n < kX100Vector) break; /* planted: the last partial vector is dropped */
+#endif
+      x100_vector(w->c, v, n, w->date, w->a, &w->unexpected);
+    }
+  }
+  return NULL;
+}
+
+void q1o_x100(const q1o_compact* c, int64_t begin, int64_t end, int32_t cutoff, int threads,
+              int64_t morsel_rows, q1o_result* out) {
+  if (threads < 1) threads = 1;
+  if (morsel_rows < 1) morsel_rows = 10 * 1024;
+  /* `const int16_t date = cmp.dte_val - 727563` (kernel_x100.hpp:147-148) */
+  const int16_t date = (int16_t)(cutoff - 727563);
+  _Atomic int64_t next = begin;
+  x100_worker_t* ws = (x100_worker_t*)calloc((size_t)threads, sizeof(x100_worker_t));
+  pthread_t* ts = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  for (int t = 0; t < threads; t++) {
+    ws[t].c = c; ws[t].end = end; ws[t].date = date; ws[t].morsel = morsel_rows; ws[t].next = &next;
+    pthread_create(&ts[t], NULL, x100_worker, &ws[t]);
+  }
+  acc_t total[6];
+  memset(total, 0, sizeof(total));
+  int64_t unexpected = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(ts[t], NULL);
+    for (int g = 0; g < 6; g++) {
+      total[g].sum_qty = (int64_t)((uint64_t)total[g].sum_qty + (uint64_t)ws[t].a[g].sum_qty);
+      total[g].sum_base_price =
+          (int64_t)((uint64_t)total[g].sum_base_price + (uint64_t)ws[t].a[g].sum_base_price);
+      total[g].sum_disc = (int64_t)((uint64_t)total[g].sum_disc + (uint64_t)ws[t].a[g].sum_disc);
+      total[g].sum_disc_price += ws[t].a[g].sum_disc_price;
+      total[g].sum_charge += ws[t].a[g].sum_charge;
+      total[g].count += ws[t].a[g].count;
+    }
+    unexpected += ws[t].unexpected;
+  }
+  free(ws);
+  free(ts);
+  acc_to_result(total, unexpected, out);
+}
+
+/* ---- AVX512<false, false, true> (cpu/kernel_avx512.hpp:11-194), lane by lane ----------- */
+/* Steps of 16 rows in two halves of 8; row j of a half updates the 8-way replicated table
+ * entry gid * 8 + j (gid = rf << 8 | ls, make_index :113-119).  The 128-bit sums are
+ * (lo, hi) word pairs updated as add64_to_int128 does (cpu/common.hpp:102-117): lo += v,
+ * then hi += 1 when the new lo is below v in a *signed* comparison.  The final per-group
+ * result adds the 8 entries (cpu/q1.cpp:86-101): 64-bit sums plainly, 128-bit sums as
+ * (hi << 64) | (int128)lo with lo sign-extended.  Below 2^63 per entry this equals the
+ * naive sums exactly; above, the reference's carry and combine rules are kept as they are. */
+enum { kAvxLanes = 8, kAvxGroups = 1 << 16 };
+
+typedef struct {
+  int64_t count, sum_qty, sum_base_price, sum_disc;
+  int64_t dp_lo, dp_hi, ch_lo, ch_hi;
+} avx_entry_t;
+
+static inline void avx_add128(int64_t* hi, int64_t* lo, int64_t v) {
+  const int64_t nlo = (int64_t)((uint64_t)*lo + (uint64_t)v);
+  if (nlo < v) *hi = (int64_t)((uint64_t)*hi + 1u);  /* _mm512_cmplt_epi64_mask: signed */
+  *lo = nlo;
+}
+
+void q1o_avx512(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff, q1o_result* out) {
+  avx_entry_t* t = (avx_entry_t*)calloc((size_t)kAvxGroups * kAvxLanes, sizeof(avx_entry_t));
+  for (int64_t off = begin; off < end; off += 16) {
+    for (int j = 0; j < 16; j++) {
+      const int64_t i = off + j;
+      if (i >= end || li->shipdate[i] > cutoff) continue; /* the select and cardinality masks */
+      const int64_t disc = li->discount[i], price = li->extendedprice[i];
+      const int64_t disc_1 = 100 - disc, tax_1 = 100 + li->tax[i];
+      const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)price);  /* mul_64_64_64 */
+      const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)tax_1);
+      const uint32_t gid = (uint32_t)(uint8_t)li->returnflag[i] << 8 | (uint8_t)li->linestatus[i];
+      avx_entry_t* e = &t[(size_t)gid * kAvxLanes + (j & 7)];
+      e->count++;
+      e->sum_qty = (int64_t)((uint64_t)e->sum_qty + (uint64_t)li->quantity[i]);
+      e->sum_base_price = (int64_t)((uint64_t)e->sum_base_price + (uint64_t)price);
+      e->sum_disc = (int64_t)((uint64_t)e->sum_disc + (uint64_t)disc);
+      avx_add128(&e->dp_hi, &e->dp_lo, disc_price);
+      avx_add128(&e->ch_hi, &e->ch_lo, charge);
+    }
+  }
+  memset(out, 0, sizeof(*out));
+  for (uint32_t gid = 0; gid < kAvxGroups; gid++) {
+    acc_t a;
+    memset(&a, 0, sizeof(a));
+#if Q1O_MUTANT == 13
+    const int lanes = kAvxLanes - 1; /* planted: the last lane is left out of the final sum */
+#else
+    const int lanes = kAvxLanes;
+#endif
+    for (int l = 0; l < lanes; l++) {
+      const avx_entry_t* e = &t[(size_t)gid * kAvxLanes + l];
+      a.count += e->count;
+      a.sum_qty = (int64_t)((uint64_t)a.sum_qty + (uint64_t)e->sum_qty);
+      a.sum_base_price = (int64_t)((uint64_t)a.sum_base_price + (uint64_t)e->sum_base_price);
+      a.sum_disc = (int64_t)((uint64_t)a.sum_disc + (uint64_t)e->sum_disc);
+      a.sum_disc_price += (i128)((unsigned __int128)(uint64_t)e->dp_hi << 64) | (i128)e->dp_lo;
+      a.sum_charge += (i128)((unsigned __int128)(uint64_t)e->ch_hi << 64) | (i128)e->ch_lo;
+    }
+    if (a.count <= 0) continue;
+    const int s = slot_of((char)(gid >> 8), (char)(gid & 0xFF));
+    if (s < 0) { out->unexpected_keys += a.count; continue; }
+    q1o_group* o = &out->g[s];
+    o->sum_qty = a.sum_qty;
+    o->sum_base_price = a.sum_base_price;
+    o->sum_disc_price_lo = (uint64_t)a.sum_disc_price;
+    o->sum_disc_price_hi = (int64_t)(a.sum_disc_price >> 64);
+    o->sum_charge_lo = (uint64_t)a.sum_charge;
+    o->sum_charge_hi = (int64_t)(a.sum_charge >> 64);
+    o->sum_disc = a.sum_disc;
+    o->count = a.count;
+  }
+  free(t);
 }

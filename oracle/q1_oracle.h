@@ -108,6 +108,31 @@ void q1o_precompute_filter(const uint16_t* shipdate, int64_t n, int32_t cutoff, 
 void q1o_pushdown_kernel(const uint32_t* filter, const q1o_compressed* c, int64_t begin,
                          int64_t end, q1o_result* out);
 
+/* ComprData (cpu/common.cpp:113-137, the -DGPU variant): the compact columns the X100
+ * kernel reads; quantity and prices keep the x100 scale. */
+typedef struct {
+  int16_t* shipdate;        /* shipdate - 727563 */
+  int8_t* returnflag;
+  int8_t* linestatus;
+  int8_t* discount;
+  int8_t* tax;
+  int32_t* extendedprice;
+  int16_t* quantity;
+} q1o_compact;
+void q1o_compact_init(const q1o_lineitem* li, int64_t n, q1o_compact* out);
+
+/* KernelX100::task (cpu/kernel_x100.hpp:140-402) with the k1Step aggregation, morsel-
+ * parallel as Morsel::Query (:456-500): vectors of 1024 rows, select on int16 dates,
+ * int8 disc_1 / tax_1, int32 disc_price, int64 charge, and sum_disc = sum of disc_1
+ * (the reference's quirk, kernel_x100.hpp:320). */
+void q1o_x100(const q1o_compact* c, int64_t begin, int64_t end, int32_t cutoff, int threads,
+              int64_t morsel_rows, q1o_result* out);
+
+/* AVX512<false, false, true> (cpu/kernel_avx512.hpp:11-194) lane by lane: an 8-way
+ * replicated table per (returnflag << 8 | linestatus), the reference's 128-bit carry rule
+ * (cpu/common.hpp:102-117) and final 8-lane sum (cpu/q1.cpp:86-101). */
+void q1o_avx512(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff, q1o_result* out);
+
 #ifdef __cplusplus
 }
 #endif

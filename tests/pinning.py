@@ -272,6 +272,41 @@ def check_parallel(o) -> bool:
     return o.naive_parallel(cols, 0, 5000, CUTOFF, 4, 100) == o.naive(cols, 0, 5000, CUTOFF)
 
 
+def check_x100(o) -> bool:
+    """KernelX100 over ComprData equals the independent formulation except for the
+    reference's sum_disc quirk: it sums disc_1 = 100 - disc, i.e. 100 * count - sum(disc).
+    Sizes and morsels leave partial 1024-row vectors inside and at the end of morsels."""
+    rng = np.random.default_rng(29)
+    for n in (1, 1023, 1024, 1025, 5003):
+        rows = random_rows(rng, n, tpch=True)
+        cols = ref_columns(rows)
+        comp = o.compact(cols, n)
+        for cut in (CUTOFF, 729906, 730088):
+            want = py_q1(rows, cut)
+            for g in want:
+                g["sum_disc"] = 100 * g["count"] - g["sum_disc"]
+            for threads, morsel in ((1, 10 * 1024), (3, 1500)):
+                got, unexpected = o.x100(comp, 0, n, cut, threads, morsel)
+                if unexpected or not _same(got, want):
+                    return False
+    return True
+
+
+def check_avx512(o) -> bool:
+    """The AVX-512 kernel's 8-lane table equals the independent formulation exactly while
+    every lane's sums stay below 2^63 (all TPC-H-domain inputs); sizes that are not
+    multiples of 16 exercise the cardinality mask."""
+    rng = np.random.default_rng(31)
+    for n in (1, 7, 15, 16, 17, 200, 1003):
+        rows = random_rows(rng, n, tpch=True)
+        cols = ref_columns(rows)
+        for cut in (CUTOFF, 729906):
+            got, unexpected = o.avx512(cols, 0, n, cut)
+            if unexpected or not _same(got, py_q1(rows, cut)):
+                return False
+    return True
+
+
 CHECKS = {name[len("check_"):]: fn for name, fn in list(globals().items()) if name.startswith("check_")}
 
 

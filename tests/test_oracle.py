@@ -17,6 +17,8 @@ MUTANTS = {
     9: "compress",           # quantity not narrowed
     10: "filter_bits",       # last filter word dropped
     11: "brute_force",       # sum_disc of (1 - disc)
+    12: "x100",              # X100 drops a morsel's last partial vector
+    13: "avx512",            # AVX-512 final sum leaves out one lane
 }
 
 
