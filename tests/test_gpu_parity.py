@@ -196,6 +196,47 @@ def test_compressed_out_of_domain(ctx, oracle):
         assert as_dicts(ctx.run(t, 0, rows, 729999, p)) == want
 
 
+@pytest.mark.parametrize("layout", [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED])
+def test_mixed_domain_tiles(ctx, oracle, layout):
+    """A table outside the TPC-H domain only in scattered rows: the any-value kernel takes
+    the narrow path for in-domain tiles and the 64-bit path for the others, within the
+    same wavefronts.  Every placement, several launch shapes, exact vs the oracle."""
+    rows = 300_007
+    ref = {k: v.copy() for k, v in host_tables(rows)[q1.REFERENCE].items()}
+    rng = np.random.default_rng(11)
+    bad = rng.choice(rows, 40, replace=False)
+    if layout == q1.COMPRESSED:
+        # compressed columns hold u8 discount/tax/quantity and u32 price
+        ref["discount"][bad[:10]] = 200
+        ref["tax"][bad[10:20]] = 99
+        ref["extended_price"][bad[20:30]] = 3_000_000_000
+        ref["quantity"][bad[30:]] = 9900        # 99 units
+        dev = q1.compress_columns(ref, rows)
+    else:
+        ref["discount"][bad[:10]] = -5
+        ref["tax"][bad[10:20]] = 1 << 40
+        ref["extended_price"][bad[20:30]] = 1 << 45
+        ref["quantity"][bad[30:]] = 6301
+        if layout == q1.UNCOMPRESSED:
+            # int32 quantity, int8 discount/tax: keep values the layout can hold
+            ref["tax"][bad[10:20]] = 100
+            dev = {"ship_date": ref["ship_date"], "quantity": ref["quantity"].astype(np.int32),
+                   "extended_price": ref["extended_price"], "discount": ref["discount"].astype(np.int8),
+                   "tax": ref["tax"].astype(np.int8), "return_flag": ref["return_flag"],
+                   "line_status": ref["line_status"]}
+        else:
+            dev = ref
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(layout, dev, rows)
+    for p in PLACEMENTS:
+        for shape in (None, (64, 16), (256, 1024)):
+            note(f"case layout={q1.LAYOUT_NAMES[layout]} placement={p} shape={shape}")
+            got = ctx.run(t, 0, rows, 729999, p, q1.make_launch(*shape) if shape else None)
+            assert as_dicts(got) == want, (p, shape)
+    assert ctx.last_timing().placement == q1.IN_REGISTERS_PER_THREAD   # the 64-bit kernel ran
+    t.free()
+
+
 def test_int128_sums_cross_2_64_general_path(ctx, oracle):
     rows = [(729999, 1, 9 * 10**14, 0, 0, "N", "O")] * 4000 + [(729999, 1, 9 * 10**14, 200, 0, "R", "F")] * 4000
     cols = pinning.ref_columns(rows)

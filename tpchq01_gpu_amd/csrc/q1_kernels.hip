@@ -134,6 +134,15 @@ struct CompressedTile {
       }
     }
   }
+  // Every row of the tile inside the TPC-H value domain (discount, tax <= 15, quantity
+  // <= 63 units, price < 2^24); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    const uint32_t dd = d.x | d.y | d.z | d.w, xx = x.x | x.y | x.z | x.w, qq = q.x | q.y | q.z | q.w;
+    uint32_t pp = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) pp |= p[k].x | p[k].y | p[k].z | p[k].w;
+    return ((dd | xx) & 0xF0F0F0F0u) == 0 && (qq & 0xC0C0C0C0u) == 0 && (pp & 0xFF000000u) == 0;
+  }
   // `cut` is the cutoff relative to the frame of reference (may be < 0 or > 65535).
   __device__ __forceinline__ bool pass(int j, int32_t cut) const {
     if (PUSHDOWN) return (flt >> j) & 1;
@@ -221,6 +230,19 @@ struct UncompressedTile {
   __device__ __forceinline__ bool pass(int j, int32_t cut) const {
     return (int32_t)comp(s[j >> 2], j & 3) <= cut;
   }
+  // Every row inside the TPC-H value domain (discount, tax in [0, 15], quantity in
+  // [0, 6300] hundredths, price in [0, 2^24)); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    uint32_t qmax = 0, phi = 0, plo = 0;
+#pragma unroll
+    for (int k = 0; k < 2; k++) qmax = max(qmax, max(max(q[k].x, q[k].y), max(q[k].z, q[k].w)));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      plo |= p[k].x | p[k].z;
+      phi |= p[k].y | p[k].w;
+    }
+    return qmax <= 6300u && phi == 0 && (plo & 0xFF000000u) == 0 && ((d.x | d.y | x.x | x.y) & 0xF0F0F0F0u) == 0;
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -297,6 +319,19 @@ struct ReferenceTile {
     return encode_return_flag((rf >> (8 * j)) & 0xFFu) << 1 | encode_line_status((ls >> (8 * j)) & 0xFFu);
   }
   __device__ __forceinline__ bool pass(int j, int32_t cut) const { return (int32_t)comp(s, j) <= cut; }
+  // Every row inside the TPC-H value domain (int64 discount, tax in [0, 15], quantity in
+  // [0, 6300], price in [0, 2^24)); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    uint32_t hi = 0, qmax = 0, plo = 0, dx = 0;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      hi |= q[k].y | q[k].w | p[k].y | p[k].w | d[k].y | d[k].w | x[k].y | x[k].w;
+      qmax = max(qmax, max(q[k].x, q[k].z));
+      plo |= p[k].x | p[k].z;
+      dx |= d[k].x | d[k].z | x[k].x | x[k].z;
+    }
+    return hi == 0 && qmax <= 6300u && (plo & 0xFF000000u) == 0 && (dx & ~15u) == 0;
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -769,8 +804,15 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
 }
 
 // ------------------------------------------------------------------------------------
-// The general (64-bit) scan kernel: per-lane register table, wrapping int64 per-row
-// arithmetic, 128-bit signed sums of disc_price and charge.
+// The any-value scan kernel (tables outside the TPC-H domain, and q1_enqueue without
+// Q1_LAUNCH_ASSUME_TPCH_DOMAIN): exact for every input with the reference's arithmetic,
+// wrapping int64 per-row products and 128-bit signed sums.  Each lane tests its tile:
+// a tile whose rows all lie inside the TPC-H value domain goes through the narrow
+// per-lane LDS table of the shared placement (exact there: no product can wrap); any
+// other tile goes through a per-lane 64-bit register table.  At the end the narrow table
+// is folded into the wide one (disc_price = 100 * price - disc * price, all
+// non-negative), and the block merges in 128 bits.  In-domain data runs at the speed of
+// the narrow kernel; a wavefront pays for the wide path only on tiles that need it.
 // ------------------------------------------------------------------------------------
 struct Wide128 {
   uint64_t lo;
@@ -782,9 +824,9 @@ struct Wide128 {
   }
 };
 
-template <int L>
-__global__ void __launch_bounds__(kMaxBlock)
-scan_general(ScanArgs a, uint32_t tiles_per_thread) {
+template <int L, int B>
+__global__ void __launch_bounds__(B)
+scan_any(ScanArgs a, uint32_t tiles_per_thread) {
   using Tile = typename TileOf<L>::T;
   constexpr int TILE = Tile::TILE;
   extern __shared__ uint64_t lds[];
@@ -792,8 +834,10 @@ scan_general(ScanArgs a, uint32_t tiles_per_thread) {
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
   const uint64_t first_tile = a.row_begin / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
-  const uint64_t base = first_tile + (uint64_t)blockIdx.x * blockDim.x * tiles_per_thread;
+  const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
 
+  AccShared<B> narrow;
+  narrow.init(reinterpret_cast<char*>(lds));
   uint64_t qty[kNumGroups], price[kNumGroups], disc[kNumGroups], cnt[kNumGroups];
   Wide128 dp[kNumGroups], ch[kNumGroups];
 #pragma unroll
@@ -803,49 +847,76 @@ scan_general(ScanArgs a, uint32_t tiles_per_thread) {
   }
 
   for (uint32_t k = 0; k < tiles_per_thread; k++) {
-    const uint64_t t = base + (uint64_t)k * blockDim.x + threadIdx.x;
+    const uint64_t t = base + (uint64_t)k * B + threadIdx.x;
     if (t >= end_tile) break;
     Tile tile;
     uint32_t valid = (1u << TILE) - 1;
-    if (t * TILE < a.row_begin || (t + 1) * TILE > a.row_end) {
+    const bool edge = t * TILE < a.row_begin || (t + 1) * TILE > a.row_end;
+    if (edge) {
       valid = valid_mask<TILE>(t, a.row_begin, a.row_end);
       tile.load_edge(a.cols, t, valid);
     } else {
       tile.load(a.cols, t);
     }
+    if (tile.in_domain()) {
+      if (!edge) {
 #pragma unroll
-    for (int j = 0; j < TILE; j++) {
-      const WideRow r = tile.wide(j, cut, valid);
-      const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
-      const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
-      const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
+        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
+      } else {
 #pragma unroll
-      for (int g = 0; g < kNumGroups; g++) {
-        if (r.gid == (uint32_t)g) {
-          qty[g] += (uint64_t)r.qty;
-          price[g] += (uint64_t)r.price;
-          disc[g] += (uint64_t)r.disc;
-          cnt[g] += 1;
-          dp[g].add64s(disc_price);
-          ch[g].add64s(charge);
+        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, valid));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < TILE; j++) {
+        const WideRow r = tile.wide(j, cut, valid);
+        const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
+        const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
+        const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
+#pragma unroll
+        for (int g = 0; g < kNumGroups; g++) {
+          if (r.gid == (uint32_t)g) {
+            qty[g] += (uint64_t)r.qty;
+            price[g] += (uint64_t)r.price;
+            disc[g] += (uint64_t)r.disc;
+            cnt[g] += 1;
+            dp[g].add64s(disc_price);
+            ch[g].add64s(charge);
+          }
         }
       }
     }
   }
 
-  // Block merge in 128 bits for every field.
-  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+  // Fold the lane's narrow table into its wide one.  Lane sums of the narrow table are
+  // exact and far below 2^63 (at most 16384 rows per lane).
+  {
+    LaneTotals nt;
+    narrow.totals(nt, reinterpret_cast<char*>(lds));
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      qty[g] += nt.v[g][F_QTY];
+      price[g] += nt.v[g][F_PRICE];
+      disc[g] += nt.v[g][F_DISC];
+      cnt[g] += nt.v[g][F_CNT];
+      dp[g].add64s((int64_t)(100u * nt.v[g][F_PRICE] - nt.v[g][F_DPROD]));
+      ch[g].add64s((int64_t)nt.v[g][F_CHARGE]);
+    }
+  }
+
+  // Block merge in 128 bits for every field; the scratch follows the narrow table.
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = B >> 6;
   constexpr int NW = 8;  // qty, price, disc, cnt, dp.lo, dp.hi, ch.lo, ch.hi
-  uint64_t* red = lds;   // nwaves * 6 * NW * 2 words
+  uint64_t* red = lds + shared_table_bytes(B) / 8;  // nwaves * 6 * NW words
 #pragma unroll
   for (int g = 0; g < kNumGroups; g++) {
     uint64_t s64[4] = {qty[g], price[g], disc[g], cnt[g]};
     Wide128 w[2] = {dp[g], ch[g]};
     if (ballot(cnt[g] != 0)) {
 #pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) {
+      for (int f = 0; f < 4; f++) s64[f] = wave_sum_u64(s64[f]);
 #pragma unroll
-        for (int f = 0; f < 4; f++) s64[f] += __shfl_xor(s64[f], m, 64);
+      for (int m = 32; m >= 1; m >>= 1) {
 #pragma unroll
         for (int f = 0; f < 2; f++) {
           const uint64_t olo = __shfl_xor(w[f].lo, m, 64), ohi = __shfl_xor(w[f].hi, m, 64);
@@ -1157,12 +1228,17 @@ FastFn fast_kernel(q1_layout l, int p, uint32_t threads) {
   }
 }
 
-GenFn general_kernel(q1_layout l) {
+template <int L>
+GenFn any_fn(uint32_t threads) {
+  return threads == 64 ? scan_any<L, 64> : threads == 128 ? scan_any<L, 128> : scan_any<L, 256>;
+}
+
+GenFn general_kernel(q1_layout l, uint32_t threads) {
   switch (l) {
-    case Q1_LAYOUT_REFERENCE: return scan_general<Q1_LAYOUT_REFERENCE>;
-    case Q1_LAYOUT_UNCOMPRESSED: return scan_general<Q1_LAYOUT_UNCOMPRESSED>;
-    case Q1_LAYOUT_COMPRESSED: return scan_general<Q1_LAYOUT_COMPRESSED>;
-    default: return scan_general<Q1_LAYOUT_COMPRESSED_PUSHDOWN>;
+    case Q1_LAYOUT_REFERENCE: return any_fn<Q1_LAYOUT_REFERENCE>(threads);
+    case Q1_LAYOUT_UNCOMPRESSED: return any_fn<Q1_LAYOUT_UNCOMPRESSED>(threads);
+    case Q1_LAYOUT_COMPRESSED: return any_fn<Q1_LAYOUT_COMPRESSED>(threads);
+    default: return any_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(threads);
   }
 }
 
@@ -1254,8 +1330,8 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
       hipLaunchKernelGGL(fold_spread, dim3(1), dim3(256), 0, s, spread->tables, args.out);
     }
   } else {
-    const uint32_t lds = nwaves * kNumGroups * 8 * 8;
-    hipLaunchKernelGGL(general_kernel(layout), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
+    const uint32_t lds = shared_table_bytes(shape.threads) + nwaves * kNumGroups * 8 * 8;
+    hipLaunchKernelGGL(general_kernel(layout, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
                        shape.tiles_per_thread);
   }
   return hip_status(hipGetLastError());
