@@ -163,6 +163,8 @@ const char* q1_last_error(q1_ctx* ctx); /* (new) detail text of the last failure
 
 /* --- context ------------------------------------------------------------------ */
 q1_status q1_init(int device, q1_ctx** out);
+/* Releases the context.  Tables of the context that are still alive lose their device
+ * memory; their handles then serve only q1_table_free. */
 void q1_shutdown(q1_ctx* ctx);
 q1_status q1_device_info_get(q1_ctx* ctx, q1_device_info* out);
 q1_status q1_synchronize(q1_ctx* ctx);
@@ -182,6 +184,7 @@ q1_status q1_table_info(const q1_table* t, q1_layout* layout, uint64_t* rows, q1
 /* (new) Copies the device columns back into caller host buffers of the layout's sizes. */
 q1_status q1_table_download(q1_ctx* ctx, const q1_table* t, const q1_columns* host_out);
 uint64_t q1_table_device_bytes(const q1_table* t);
+/* Frees the table's device memory (if its context is still alive) and the handle. */
 void q1_table_free(q1_table* t);
 /* Pushdown producer on the device: builds a COMPRESSED_PUSHDOWN table sharing the
  * compressed table's columns plus a filter bitmap for `cutoff_day`.  The pushdown table

@@ -112,6 +112,19 @@ def test_wrap_torch_columns(ctx, oracle):
         ctx.wrap(q1.UNCOMPRESSED, c, rows - 1)
 
 
+def test_table_outliving_its_context():
+    """Tables still alive when their context shuts down lose their device memory; their
+    handles stay valid for q1_table_free (what Python's garbage collector may do last)."""
+    c = q1.Context(0)
+    t = c.generate(q1.COMPRESSED, SEED, 10_000)
+    pd = c.build_filter(t, 729999)
+    u = c.generate(q1.REFERENCE, SEED, 1000)
+    u.free()
+    c.close()
+    pd.free()
+    t.free()
+
+
 def test_errors(ctx):
     t = ctx.generate(q1.COMPRESSED, SEED, 1000)
     with pytest.raises(q1.Q1Error):

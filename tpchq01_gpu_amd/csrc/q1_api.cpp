@@ -170,6 +170,18 @@ static void load_tuning_file(q1_ctx* ctx) {
   }
 }
 
+// Frees a table with its context's lock held (every entry point that creates tables
+// holds it when a creation fails).
+static void release_table(q1_table* t) {
+  if (q1_ctx* ctx = t->ctx) {
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : t->owned)
+      if (p) (void)hipFree(p);
+    ctx->tables.erase(t);
+  }
+  delete t;
+}
+
 }  // namespace q1
 
 using namespace q1;
@@ -241,6 +253,18 @@ q1_status q1_init(int device, q1_ctx** out) {
 void q1_shutdown(q1_ctx* ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  // Tables still alive lose their device memory; their handles stay valid for
+  // q1_table_free only.
+  for (q1_table* t : ctx->tables) {
+    for (void*& p : t->owned) {
+      if (p) (void)hipFree(p);
+      p = nullptr;
+    }
+    t->ctx = nullptr;
+    t->dev = q1_columns{};
+    t->rows = 0;
+  }
+  ctx->tables.clear();
   if (ctx->batch) batch_destroy(ctx->batch);
   stream_state_release(ctx);
   if (ctx->workspace) (void)hipFree(ctx->workspace);
@@ -296,12 +320,13 @@ static q1_status alloc_table(q1_ctx* ctx, q1_layout layout, uint64_t rows, q1_ta
   t->ctx = ctx;
   t->layout = layout;
   t->rows = rows;
+  ctx->tables.insert(t);
   for (int i = 0; i < 8; i++) {
     if (!b[i]) continue;
     void* p = nullptr;
     const hipError_t e = hipMalloc(&p, b[i]);
     if (e != hipSuccess) {
-      q1_table_free(t);
+      release_table(t);
       return fail(ctx, Q1_ENOMEM, "device allocation of a table column failed");
     }
     t->owned[i] = p;
@@ -335,7 +360,7 @@ q1_status q1_table_upload(q1_ctx* ctx, q1_layout layout, const q1_columns* host,
   ok = ok && hipEventRecord(ctx->ev_h2d_end, ctx->stream) == hipSuccess;
   ok = hipStreamSynchronize(ctx->stream) == hipSuccess && ok;
   if (!ok) {
-    q1_table_free(t);
+    release_table(t);
     return fail(ctx, Q1_EDEVICE, "upload of the table columns failed");
   }
   float ms = 0;
@@ -355,7 +380,7 @@ q1_status q1_table_upload(q1_ctx* ctx, q1_layout layout, const q1_columns* host,
     s = measure_domain(ctx, t);
   }
   if (s != Q1_OK) {
-    q1_table_free(t);
+    release_table(t);
     return s;
   }
   *out = t;
@@ -393,6 +418,7 @@ q1_status q1_table_wrap(q1_ctx* ctx, q1_layout layout, const q1_columns* device,
   }
   t->domain = tmp.domain;
   t->fast = tmp.fast;
+  ctx->tables.insert(t);
   *out = t;
   return Q1_OK;
   GUARD_END(ctx)
@@ -411,7 +437,7 @@ q1_status q1_table_generate(q1_ctx* ctx, q1_layout layout, uint64_t seed, uint64
   s = launch_generate(layout, seed, rows, t->dev, ctx->stream);
   if (s == Q1_OK) s = measure_domain(ctx, t);
   if (s != Q1_OK) {
-    q1_table_free(t);
+    release_table(t);
     return fail(ctx, s, "generator failed");
   }
   *out = t;
@@ -444,12 +470,16 @@ q1_status q1_table_download(q1_ctx* ctx, const q1_table* t, const q1_columns* ho
 
 uint64_t q1_table_device_bytes(const q1_table* t) { return t ? t->device_bytes : 0; }
 
+// A table freed after its context was shut down has already lost its device memory
+// (q1_shutdown releases it); only the handle is deleted then.
 void q1_table_free(q1_table* t) {
   if (!t) return;
-  if (t->ctx && t->ctx->stream) (void)hipStreamSynchronize(t->ctx->stream);
-  for (void* p : t->owned)
-    if (p) (void)hipFree(p);
-  delete t;
+  if (q1_ctx* ctx = t->ctx) {
+    std::lock_guard<std::mutex> g(ctx->lock);
+    release_table(t);
+  } else {
+    delete t;
+  }
 }
 
 q1_status q1_build_filter(q1_ctx* ctx, const q1_table* compressed, int32_t cutoff_day, q1_table** out) {
@@ -467,25 +497,28 @@ q1_status q1_build_filter(q1_ctx* ctx, const q1_table* compressed, int32_t cutof
   t->domain = compressed->domain;
   t->fast = compressed->fast;
   t->filter_cutoff = cutoff_day;
+  ctx->tables.insert(t);
   const uint64_t words = (t->rows + 31) / 32;
   if (words) {
     void* f = nullptr;
     if (hipMalloc(&f, words * 4) != hipSuccess) {
+      ctx->tables.erase(t);
       delete t;
       return fail(ctx, Q1_ENOMEM, "filter allocation failed");
     }
     t->owned[7] = f;
     t->dev.filter = f;
     t->device_bytes = words * 4;
-    Q1_HIP(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
-    const q1_status s = launch_build_filter((const uint16_t*)compressed->dev.ship_date, t->rows, cutoff_day,
-                                            (uint32_t*)f, ctx->stream);
-    Q1_HIP(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
+    q1_status s = hipEventRecord(ctx->ev_begin, ctx->stream) == hipSuccess ? Q1_OK : Q1_EDEVICE;
+    if (s == Q1_OK)
+      s = launch_build_filter((const uint16_t*)compressed->dev.ship_date, t->rows, cutoff_day, (uint32_t*)f,
+                              ctx->stream);
+    if (s == Q1_OK && hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess) s = Q1_EDEVICE;
+    if (s == Q1_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) s = Q1_EDEVICE;
     if (s != Q1_OK) {
-      q1_table_free(t);
-      return fail(ctx, s, "filter kernel launch failed");
+      release_table(t);
+      return fail(ctx, s, "building the filter bitmap failed");
     }
-    Q1_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);
     ctx->last = q1_timing{};

@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <unordered_set>
 
 #include "../../include/q1_hip.h"
 #include "q1_kernels.h"
@@ -45,6 +46,7 @@ struct q1_ctx {
   q1::StreamState* streaming = nullptr;    // cached staging buffers and streams of q1_run_streaming
   q1::Spread spread{};                     // spread-merge tables of the context stream
   bool spread_dirty = false;               // a failed launch may have left partials in them
+  std::unordered_set<q1_table*> tables;    // live tables (released device memory at shutdown)
   std::mutex lock;                         // serialises calls that share the context stream
 };
 
