@@ -1,10 +1,11 @@
 // gfx950 kernels of the TPC-H Q1 scan -> filter -> group-aggregate path.
 //
 // Work decomposition (all layouts, all placements):
-//   * rows are cut into lane tiles of TILE consecutive rows (4 for the compressed layouts
-//     and the reference layout, 8 for the uncompressed one); for the compressed layouts
-//     every load instruction of a wavefront then reads whole cache lines of its column,
-//     which lets the column loads be non-temporal;
+//   * rows are cut into lane tiles of TILE consecutive rows (16 for the compressed layouts,
+//     8 for the uncompressed one, 4 for the reference layout); the shared placement reads
+//     the compressed layouts in 4-row tiles instead, four in flight per lane, so that every
+//     load instruction of a wavefront reads whole cache lines and the loads can be
+//     non-temporal;
 //   * a block owns tiles_per_thread * blockDim consecutive tiles; in iteration k thread i
 //     takes tile base + k*blockDim + i, so each load instruction of a wavefront covers one
 //     contiguous stretch of every column (coalesced 16-byte loads);
@@ -98,14 +99,114 @@ __device__ __forceinline__ uint2 nt_load(const uint2* p) {
 }
 __device__ __forceinline__ uint32_t nt_load(const uint32_t* p) { return __builtin_nontemporal_load(p); }
 
-// Q1_LAYOUT_COMPRESSED / Q1_LAYOUT_COMPRESSED_PUSHDOWN: 4 rows per tile, so that each load
+// Q1_LAYOUT_COMPRESSED / Q1_LAYOUT_COMPRESSED_PUSHDOWN: 16 rows per tile.
+template <bool PUSHDOWN>
+struct CompressedTile {
+  static constexpr int TILE = 16;
+  static constexpr int GROUP = 1;
+  uint4 sd[2];      // 16 x uint16 ship dates (unused for pushdown)
+  uint4 q, d, x;    // 16 x uint8 quantity, discount, tax
+  uint4 p[4];       // 16 x uint32 extended price
+  uint32_t rf;      // 16 x 2-bit return flag codes
+  uint32_t ls;      // 16 x 1-bit line status codes (low half-word)
+  uint32_t flt;     // 16 x filter bits (pushdown)
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    if (!PUSHDOWN) {
+      const uint4* s = reinterpret_cast<const uint4*>(c.ship_date);
+      sd[0] = s[2 * t];
+      sd[1] = s[2 * t + 1];
+    } else {
+      flt = reinterpret_cast<const uint16_t*>(c.filter)[t];
+    }
+    q = reinterpret_cast<const uint4*>(c.quantity)[t];
+    d = reinterpret_cast<const uint4*>(c.discount)[t];
+    x = reinterpret_cast<const uint4*>(c.tax)[t];
+    const uint4* pp = reinterpret_cast<const uint4*>(c.extended_price);
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = pp[4 * t + k];
+    rf = reinterpret_cast<const uint32_t*>(c.return_flag)[t];
+    ls = reinterpret_cast<const uint16_t*>(c.line_status)[t];
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    sd[0] = sd[1] = q = d = x = p[0] = p[1] = p[2] = p[3] = make_uint4(0, 0, 0, 0);
+    rf = ls = flt = 0;
+    // The packed words of a tile hold rows of the tile only, so reading them for a tile
+    // that has at least one valid row stays inside the column.
+    if (valid) {
+      rf = reinterpret_cast<const uint32_t*>(c.return_flag)[t];
+      ls = reinterpret_cast<const uint16_t*>(c.line_status)[t];
+      if (PUSHDOWN) flt = reinterpret_cast<const uint16_t*>(c.filter)[t];
+    }
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        if (!PUSHDOWN) {
+          const uint32_t s = reinterpret_cast<const uint16_t*>(c.ship_date)[r];
+          comp_or(sd[j >> 3], (j >> 1) & 3, s << (16 * (j & 1)));
+        }
+        comp_or(q, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.quantity)[r] << (8 * (j & 3)));
+        comp_or(d, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.discount)[r] << (8 * (j & 3)));
+        comp_or(x, j >> 2, (uint32_t)reinterpret_cast<const uint8_t*>(c.tax)[r] << (8 * (j & 3)));
+        comp_set(p[j >> 2], j & 3, reinterpret_cast<const uint32_t*>(c.extended_price)[r]);
+      }
+    }
+  }
+  // Every row of the tile inside the TPC-H value domain (discount, tax <= 15, quantity
+  // <= 63 units, price < 2^24); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    const uint32_t dd = d.x | d.y | d.z | d.w, xx = x.x | x.y | x.z | x.w, qq = q.x | q.y | q.z | q.w;
+    uint32_t pp = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) pp |= p[k].x | p[k].y | p[k].z | p[k].w;
+    return ((dd | xx) & 0xF0F0F0F0u) == 0 && (qq & 0xC0C0C0C0u) == 0 && (pp & 0xFF000000u) == 0;
+  }
+  // `cut` is the cutoff relative to the frame of reference (may be < 0 or > 65535).
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const {
+    if (PUSHDOWN) return (flt >> j) & 1;
+    const uint32_t s = (comp(sd[j >> 3], (j >> 1) & 3) >> (16 * (j & 1))) & 0xFFFFu;
+    return (int32_t)s <= cut;
+  }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return __builtin_amdgcn_ubfe(rf, 2 * j, 2) << 1 | __builtin_amdgcn_ubfe(ls, j, 1);
+  }
+  __device__ __forceinline__ uint32_t byte(const uint4& v, int j) const {
+    return (comp(v, j >> 2) >> (8 * (j & 3))) & 0xFFu;
+  }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = byte(q, j);
+    r.disc = byte(d, j);
+    r.tax = byte(x, j);
+    r.price = comp(p[j >> 2], j & 3);
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = byte(q, j);
+    r.disc = byte(d, j);
+    r.tax = byte(x, j);
+    r.price = comp(p[j >> 2], j & 3);
+    return r;
+  }
+};
+
+// Streaming variant for the compressed layouts (used with the shared placement): 4 rows
+// per tile, so that each load
 // instruction of a wavefront reads one contiguous, line-aligned stretch of its column:
 // 64 lanes x 8 bytes of ship dates, x 4 bytes of quantity / discount / tax, x 16 bytes of
 // prices.  The 4 rows' return flags are one byte of the packed column (2 bits per row)
 // and their line statuses / filter bits one nibble (1 bit per row).
 template <bool PUSHDOWN>
-struct CompressedTile {
+struct CompressedStreamTile {
   static constexpr int TILE = 4;
+  static constexpr int GROUP = 4;  // tiles loaded together per lane
   uint2 sd;         // 4 x uint16 ship dates (unused for pushdown)
   uint32_t q, d, x; // 4 x uint8 quantity, discount, tax
   uint4 p;          // 4 x uint32 extended price
@@ -197,6 +298,7 @@ struct CompressedTile {
 // Q1_LAYOUT_UNCOMPRESSED: int32 ship, int32 qty, int64 price, int8 disc/tax, char rf/ls.
 struct UncompressedTile {
   static constexpr int TILE = 8;
+  static constexpr int GROUP = 1;
   uint4 s[2], q[2], p[4];
   uint2 d, x, rf, ls;
 
@@ -285,6 +387,7 @@ struct UncompressedTile {
 // Q1_LAYOUT_REFERENCE: int32 ship, int64 qty/price/disc/tax, char rf/ls.
 struct ReferenceTile {
   static constexpr int TILE = 4;
+  static constexpr int GROUP = 1;
   uint4 s, q[2], p[2], d[2], x[2];
   uint32_t rf, ls;
 
@@ -376,6 +479,55 @@ template <> struct TileOf<Q1_LAYOUT_REFERENCE> { using T = ReferenceTile; };
 template <> struct TileOf<Q1_LAYOUT_UNCOMPRESSED> { using T = UncompressedTile; };
 template <> struct TileOf<Q1_LAYOUT_COMPRESSED> { using T = CompressedTile<false>; };
 template <> struct TileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN> { using T = CompressedTile<true>; };
+
+// The tile a scan_fast<L, P> kernel reads.  With the compressed layouts the LDS placement
+// (little register state per lane) streams 4-row tiles with non-temporal loads, four tiles
+// in flight per lane; the other placements keep TileOf<L>, whose tile sizes are also the
+// units the host counts tiles in (tile_rows).
+template <int L, int P> struct ScanTileOf { using T = typename TileOf<L>::T; };
+
+// A group of U lane tiles t0 + m * stride (m < U, at most `count` of them, below
+// `end_tile`), all loaded before any of their rows is used, so that a wavefront keeps
+// 16 rows' worth of loads per lane in flight whatever the tile size.
+template <class Tile, int U>
+struct TileGroup {
+  Tile tile[U];
+  uint32_t valid[U];  // rows of the tile inside [row_begin, row_end)
+  bool use[U], edge[U];
+  bool plain;         // every tile used and whole: loaded without any branch
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t0, uint64_t stride, uint32_t count,
+                                       uint64_t end_tile, uint64_t row_begin, uint64_t row_end) {
+    constexpr int TILE = Tile::TILE;
+    const uint64_t t_last = t0 + (uint64_t)(U - 1) * stride;
+    plain = count >= (uint32_t)U && t_last < end_tile && t0 * TILE >= row_begin && (t_last + 1) * TILE <= row_end;
+    if (plain) {
+      // straight-line loads of all U tiles: every load of the group is in flight at once
+#pragma unroll
+      for (int m = 0; m < U; m++) {
+        tile[m].load(c, t0 + (uint64_t)m * stride);
+        use[m] = true;
+        edge[m] = false;
+        valid[m] = (1u << TILE) - 1;
+      }
+      return;
+    }
+#pragma unroll
+    for (int m = 0; m < U; m++) {
+      const uint64_t t = t0 + (uint64_t)m * stride;
+      use[m] = (uint32_t)m < count && t < end_tile;
+      edge[m] = t * TILE < row_begin || (t + 1) * TILE > row_end;
+      valid[m] = (1u << TILE) - 1;
+      if (use[m]) {
+        if (!edge[m]) {
+          tile[m].load(c, t);
+        } else {
+          valid[m] = valid_mask<TILE>(t, row_begin, row_end);
+          tile[m].load_edge(c, t, valid[m]);
+        }
+      }
+    }
+  }
+};
 
 // Cutoff in the units of the layout's ship_date column.
 template <int L>
@@ -659,6 +811,13 @@ template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
 constexpr int kPlaceSharedNarrow = 8;
 template <int B> struct AccOf<kPlaceSharedNarrow, B> { using A = AccShared<B, true>; };
 
+template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED, Q1_PLACE_PER_THREAD_SHARED_MEM> { using T = CompressedStreamTile<false>; };
+template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED, kPlaceSharedNarrow> { using T = CompressedStreamTile<false>; };
+template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN, Q1_PLACE_PER_THREAD_SHARED_MEM> {
+  using T = CompressedStreamTile<true>;
+};
+template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN, kPlaceSharedNarrow> { using T = CompressedStreamTile<true>; };
+
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
 // ------------------------------------------------------------------------------------
@@ -769,14 +928,20 @@ __device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggreg
 // ------------------------------------------------------------------------------------
 template <int L, int P, int B>
 __global__ void __launch_bounds__(B)
-scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
-  using Tile = typename TileOf<L>::T;
+scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
+  using Tile = typename ScanTileOf<L, P>::T;
   using Acc = typename AccOf<P, B>::A;
   constexpr int TILE = Tile::TILE;
-  constexpr int FLUSH = 16 / TILE;  // tiles between narrow-accumulator flushes
+  constexpr int U = Tile::GROUP;                       // tiles loaded together per lane
+  constexpr int FLUSH = U * TILE >= 16 ? 1 : 16 / (U * TILE);  // groups per 16 rows
+  // The host counts tiles of TileOf<L> (tile_rows); a smaller kernel tile splits each.
+  constexpr int SUB = TileOf<L>::T::TILE / TILE;
   extern __shared__ uint64_t lds[];
 
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
+  // Block b covers kernel tiles [first_tile + b*B*tpt, +B*tpt): SUB times the host's
+  // tiles per lane, so the grid covers every row of the range.
+  const uint32_t tiles_per_thread = host_tiles_per_thread * SUB;
   const uint64_t first_tile = a.row_begin / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
   const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
@@ -785,23 +950,29 @@ scan_fast(ScanArgs a, uint32_t tiles_per_thread, uint64_t* ws) {
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
   acc.init(reinterpret_cast<char*>(lds));
 
-  for (uint32_t k = 0; k < tiles_per_thread; k++) {
-    const uint64_t t = base + (uint64_t)k * B + threadIdx.x;
-    if (t < end_tile) {
-      Tile tile;
-      const bool edge = t * TILE < a.row_begin || (t + 1) * TILE > a.row_end;
-      if (!edge) {
-        tile.load(a.cols, t);
+  for (uint32_t k = 0, n = 0; k < tiles_per_thread; k += U, n++) {
+    TileGroup<Tile, U> grp;
+    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+             a.row_end);
+    if (grp.plain) {
 #pragma unroll
-        for (int j = 0; j < TILE; j++) acc.add(tile.template fast<false>(j, cut, 0));
-      } else {
-        const uint32_t valid = valid_mask<TILE>(t, a.row_begin, a.row_end);
-        tile.load_edge(a.cols, t, valid);
+      for (int m = 0; m < U; m++)
 #pragma unroll
-        for (int j = 0; j < TILE; j++) acc.add(tile.template fast<true>(j, cut, valid));
+        for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+    } else {
+#pragma unroll
+      for (int m = 0; m < U; m++) {
+        if (!grp.use[m]) continue;
+        if (!grp.edge[m]) {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+        } else {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
+        }
       }
     }
-    if ((k + 1) % FLUSH == 0) acc.flush();
+    if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
   }
   acc.flush();
 
@@ -862,42 +1033,40 @@ scan_any(ScanArgs a, uint32_t tiles_per_thread) {
     dp[g].lo = dp[g].hi = ch[g].lo = ch[g].hi = 0;
   }
 
-  for (uint32_t k = 0; k < tiles_per_thread; k++) {
-    const uint64_t t = base + (uint64_t)k * B + threadIdx.x;
-    if (t >= end_tile) break;
-    Tile tile;
-    uint32_t valid = (1u << TILE) - 1;
-    const bool edge = t * TILE < a.row_begin || (t + 1) * TILE > a.row_end;
-    if (edge) {
-      valid = valid_mask<TILE>(t, a.row_begin, a.row_end);
-      tile.load_edge(a.cols, t, valid);
-    } else {
-      tile.load(a.cols, t);
-    }
-    if (tile.in_domain()) {
-      if (!edge) {
+  constexpr int U = Tile::GROUP;
+  for (uint32_t k = 0; k < tiles_per_thread; k += U) {
+    TileGroup<Tile, U> grp;
+    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+             a.row_end);
 #pragma unroll
-        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
+    for (int m = 0; m < U; m++) {
+      if (!grp.use[m]) continue;
+      const Tile& tile = grp.tile[m];
+      if (tile.in_domain()) {
+        if (!grp.edge[m]) {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
+        } else {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, grp.valid[m]));
+        }
       } else {
 #pragma unroll
-        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, valid));
-      }
-    } else {
+        for (int j = 0; j < TILE; j++) {
+          const WideRow r = tile.wide(j, cut, grp.valid[m]);
+          const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
+          const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
+          const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
 #pragma unroll
-      for (int j = 0; j < TILE; j++) {
-        const WideRow r = tile.wide(j, cut, valid);
-        const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
-        const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
-        const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
-#pragma unroll
-        for (int g = 0; g < kNumGroups; g++) {
-          if (r.gid == (uint32_t)g) {
-            qty[g] += (uint64_t)r.qty;
-            price[g] += (uint64_t)r.price;
-            disc[g] += (uint64_t)r.disc;
-            cnt[g] += 1;
-            dp[g].add64s(disc_price);
-            ch[g].add64s(charge);
+          for (int g = 0; g < kNumGroups; g++) {
+            if (r.gid == (uint32_t)g) {
+              qty[g] += (uint64_t)r.qty;
+              price[g] += (uint64_t)r.price;
+              disc[g] += (uint64_t)r.disc;
+              cnt[g] += 1;
+              dp[g].add64s(disc_price);
+              ch[g].add64s(charge);
+            }
           }
         }
       }
@@ -1143,7 +1312,8 @@ batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
   using Tile = typename TileOf<L>::T;
   using Acc = typename AccOf<P, kBatchBlock>::A;
   constexpr int TILE = Tile::TILE;
-  constexpr int FLUSH = 16 / TILE;
+  constexpr int U = Tile::GROUP;                       // tiles loaded together per lane
+  constexpr int FLUSH = U * TILE >= 16 ? 1 : 16 / (U * TILE);  // groups per 16 rows
   constexpr uint64_t kMaxTilesPerLane = 16384 / TILE;  // keeps lane partials exact
   extern __shared__ uint64_t lds[];
   __shared__ uint32_t first_req;
@@ -1181,25 +1351,32 @@ batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
     const int32_t cut = column_cutoff<L>(d.cutoff_day);
     Acc acc;
     acc.init(reinterpret_cast<char*>(lds));
-    uint32_t k = 0;
-    for (uint64_t b = pos; b < seg_end; b += blockDim.x, k++) {
-      const uint64_t g = b + threadIdx.x;
-      if (g < seg_end) {
-        const uint64_t t = first_tile + (g - d.tile_begin);
-        Tile tile;
-        const bool edge = t * TILE < d.row_begin || (t + 1) * TILE > d.row_end;
-        if (!edge) {
-          tile.load(d.cols, t);
+    // tile g of the concatenated space is tile first_tile + (g - tile_begin) of the request
+    const uint64_t seg_end_tile = first_tile + (seg_end - d.tile_begin);
+    uint32_t n = 0;
+    for (uint64_t b = pos; b < seg_end; b += (uint64_t)U * kBatchBlock, n++) {
+      TileGroup<Tile, U> grp;
+      grp.load(d.cols, first_tile + (b + threadIdx.x - d.tile_begin), kBatchBlock, U, seg_end_tile, d.row_begin,
+               d.row_end);
+      if (grp.plain) {
 #pragma unroll
-          for (int j = 0; j < TILE; j++) acc.add(tile.template fast<false>(j, cut, 0));
-        } else {
-          const uint32_t valid = valid_mask<TILE>(t, d.row_begin, d.row_end);
-          tile.load_edge(d.cols, t, valid);
+        for (int m = 0; m < U; m++)
 #pragma unroll
-          for (int j = 0; j < TILE; j++) acc.add(tile.template fast<true>(j, cut, valid));
+          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+      } else {
+#pragma unroll
+        for (int m = 0; m < U; m++) {
+          if (!grp.use[m]) continue;
+          if (!grp.edge[m]) {
+#pragma unroll
+            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+          } else {
+#pragma unroll
+            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
+          }
         }
       }
-      if ((k + 1) % FLUSH == 0) acc.flush();
+      if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
     }
     acc.flush();
     LaneTotals tot;
@@ -1259,7 +1436,7 @@ GenFn general_kernel(q1_layout l, uint32_t threads) {
 }
 
 uint32_t tile_rows(q1_layout l) {
-  return l == Q1_LAYOUT_UNCOMPRESSED ? 8u : 4u;
+  return l == Q1_LAYOUT_REFERENCE ? 4u : (l == Q1_LAYOUT_UNCOMPRESSED ? 8u : 16u);
 }
 
 q1_status hip_status(hipError_t e) { return e == hipSuccess ? Q1_OK : Q1_EDEVICE; }
