@@ -168,7 +168,17 @@ def batch_leg(q1, ctx, n: int):
             "speedup": one / batched, "exact_vs_single": exact}
 
 
-def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5):
+def probe_ceiling(q1, probe, layout):
+    """The probe's best pure-read rate for a layout's columns (GB/s), or None: it reads the
+    compressed, uncompressed and reference columns with their kernels' access patterns."""
+    if not probe:
+        return None
+    key = {q1.COMPRESSED: "best_columns_gbps", q1.UNCOMPRESSED: "best_uncompressed_gbps",
+           q1.REFERENCE: "best_reference_gbps"}.get(layout)
+    return probe.get(key) if key else None
+
+
+def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5, probe=None):
     """The same rows in the other layouts (pushdown bitmap built on the device from the
     compressed table; uncompressed and reference layouts generated from the same seed):
     kernel time of q1_run with the tuned placement, mean of `reps` runs after one warm-up,
@@ -192,6 +202,10 @@ def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5):
                      "gbps": algo / (ms * 1e-3) / 1e9, "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "rows_s": rows / (ms * 1e-3), "placement": q1.PLACEMENT_NAMES[tm.placement],
                      "threads_per_block": tm.threads_per_block, "tuples_per_thread": tm.tuples_per_thread}
+        ceiling = probe_ceiling(q1, probe, t.layout)
+        if ceiling:
+            out[name]["read_ceiling_gbps"] = ceiling
+            out[name]["frac_of_read_ceiling"] = out[name]["gbps"] / ceiling
         t.free()
     out["exact_vs_compressed"] = exact
     return out
@@ -332,8 +346,7 @@ def main():
     algo_bytes = rows * q1.BITS_PER_ROW[layout] // 8
     achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
     traffic = traffic_from_profile(args.traffic_file) if layout == q1.COMPRESSED else None
-    # the probe reads the compressed columns with the scan's pattern; other layouts get its flat read
-    ceiling = (probe["best_columns_gbps"] if layout == q1.COMPRESSED else probe["best_flat_gbps"]) if probe else None
+    ceiling = probe_ceiling(q1, probe, layout)
     out = {
         "metric": METRIC,
         "value": rows * args.steps / elapsed,
@@ -361,7 +374,7 @@ def main():
     }
     if args.layouts and layout == q1.COMPRESSED:
         log("bench: layouts leg")
-        out["layouts"] = layouts_leg(q1, ctx, table, rows, 0x51C00002)
+        out["layouts"] = layouts_leg(q1, ctx, table, rows, 0x51C00002, probe=probe)
         # the same columns through the any-value kernel that runs outside the TPC-H value domain
         # (q1_enqueue without Q1_LAUNCH_ASSUME_TPCH_DOMAIN), same accumulate-and-compare check
         wide = q1.make_launch(256, 1024, 0, 0)

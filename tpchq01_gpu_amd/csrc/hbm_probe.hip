@@ -5,8 +5,9 @@
 // the scan kernel's own access pattern and launch shapes, but with no filter and no
 // aggregation (each lane only XORs what it loads), and then the same number of bytes as
 // one flat buffer, and with the streaming pattern of the shared placement (4-row tiles,
-// non-temporal loads).  Each shape is timed as 20 back-to-back launches, as bench.py times
-// the scan.  The fastest of these is the rate the scan could reach if its arithmetic
+// non-temporal loads); then the columns of the uncompressed (20 B/row) and reference
+// (38 B/row) layouts with the patterns of their kernels.  Each shape is timed as 20
+// back-to-back launches, as bench.py times the scan.  The fastest of these is the rate the scan could reach if its arithmetic
 // were free; the scan's own rate is judged against it and against the published 8 TB/s.
 //
 //   hbm_probe [rows]     (default 600000000; prints one JSON line)
@@ -84,6 +85,73 @@ __global__ __launch_bounds__(B) void read_columns_nt(Columns c, uint64_t tiles4,
   if (acc == kSentinel) sink[0] = acc;
 }
 
+// The wide layouts' columns: lane tiles of TILE rows, column k holding W[k] bytes per
+// tile.  With TILE = 2 every column load of a wavefront is one contiguous stretch with at
+// most 16 bytes per lane, loaded non-temporally (the pattern of the LDS placement's
+// kernels); with TILE = 8 (uncompressed) or 4 (reference) and ordinary loads it is the
+// pattern of the register placements.
+template <int W, bool NT>
+__device__ __forceinline__ uint32_t load_bytes(const char* col, uint64_t t) {
+  const char* p = col + t * W;
+  if constexpr (W == 2) {
+    return NT ? __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p)) : *reinterpret_cast<const uint16_t*>(p);
+  } else if constexpr (W == 4) {
+    return NT ? __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(p)) : *reinterpret_cast<const uint32_t*>(p);
+  } else if constexpr (W == 8) {
+    const u32x2_t v = NT ? __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(p))
+                         : *reinterpret_cast<const u32x2_t*>(p);
+    return v.x ^ v.y;
+  } else {
+    uint32_t a = 0;
+#pragma unroll
+    for (int k = 0; k < W / 16; k++) {
+      const u32x4_t v = NT ? __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p) + k)
+                           : reinterpret_cast<const u32x4_t*>(p)[k];
+      a ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    return a;
+  }
+}
+
+struct WideColumns {
+  const char* col[7];
+};
+
+template <bool NT, int... W>
+struct WideTile {
+  static constexpr int kColumns = sizeof...(W);
+  static constexpr int kBytes = (W + ...);
+  static __device__ __forceinline__ uint32_t load(const WideColumns& c, uint64_t t) {
+    int i = 0;
+    uint32_t a = 0;
+    ((a ^= load_bytes<W, NT>(c.col[i++], t)), ...);
+    return a;
+  }
+};
+
+// G tiles per lane in flight: lane i of the block reads tiles t0 + i + m*B, m < G.
+template <int B, int G, typename T>
+__global__ __launch_bounds__(B) void read_wide(WideColumns c, uint64_t tiles, uint32_t tiles_per_thread,
+                                               uint32_t* sink) {
+  const uint64_t begin = (uint64_t)blockIdx.x * B * tiles_per_thread;
+  const uint64_t end = begin + (uint64_t)B * tiles_per_thread < tiles ? begin + (uint64_t)B * tiles_per_thread : tiles;
+  uint32_t acc = 0;
+  for (uint32_t k = 0; k < tiles_per_thread; k += G) {
+    const uint64_t t0 = begin + (uint64_t)k * B + threadIdx.x;
+    if (t0 + (uint64_t)(G - 1) * B < end) {
+      uint32_t v[G];
+#pragma unroll
+      for (int m = 0; m < G; m++) v[m] = T::load(c, t0 + (uint64_t)m * B);
+#pragma unroll
+      for (int m = 0; m < G; m++) acc += v[m];
+    } else {
+      for (int m = 0; m < G; m++)
+        if (t0 + (uint64_t)m * B < end) acc += T::load(c, t0 + (uint64_t)m * B);
+    }
+  }
+  if (acc == kSentinel) sink[0] = acc;
+}
+
 // One flat buffer, 16-byte loads, each block a contiguous range.
 template <int B>
 __global__ __launch_bounds__(B) void read_flat(const uint4* a, uint64_t n, uint32_t per_thread, uint32_t* sink) {
@@ -115,6 +183,25 @@ double back_to_back_ms(Launch launch, int reps) {
   CHECK(hipEventDestroy(e0));
   CHECK(hipEventDestroy(e1));
   return ms / reps;
+}
+
+// One wide-layout pattern: columns of W... bytes per TILE-row tile carved one after another
+// from `base`, B threads x rows_per_thread rows per block.  Returns ms per pass.
+template <int B, int G, int TILE, bool NT, int... W>
+double wide_pass(char* base, uint64_t rows, uint32_t rows_per_thread, uint32_t* sink, int reps) {
+  using T = WideTile<NT, W...>;
+  const int widths[] = {W...};
+  WideColumns c{};
+  const uint64_t tiles = rows / TILE;
+  uint64_t off = 0;
+  for (int i = 0; i < T::kColumns; i++) {
+    c.col[i] = base + off;
+    off += (tiles * (uint64_t)widths[i] + 255) / 256 * 256;
+  }
+  const uint32_t tpt = rows_per_thread / TILE;
+  const uint64_t blocks = (tiles + (uint64_t)B * tpt - 1) / ((uint64_t)B * tpt);
+  return back_to_back_ms([&] { read_wide<B, G, T><<<dim3((uint32_t)blocks), dim3(B)>>>(c, tiles, tpt, sink); },
+                         reps);
 }
 
 }  // namespace
@@ -204,9 +291,34 @@ int main(int argc, char** argv) {
                 ms, n16 * 16 / ms / 1e6);
   }
   best_cols = std::min(best_cols, best_stream);
-  std::printf("], \"best_columns_gbps\": %.1f, \"best_streaming_gbps\": %.1f, \"best_flat_gbps\": %.1f}\n",
-              bytes / best_cols / 1e6, bytes / best_stream / 1e6, n16 * 16 / best_flat / 1e6);
   CHECK(hipFree(base));
+
+  // The wide layouts over the same rows: uncompressed (20 B/row: int32 ship, int32 qty,
+  // int64 price, int8 disc, tax, char rf, ls) and reference (38 B/row: int32 ship, int64
+  // qty, price, disc, tax, char rf, ls).
+  char* wide = nullptr;
+  CHECK(hipMalloc(&wide, rows * 38 + 8 * 256));
+  CHECK(hipMemset(wide, 0x5A, rows * 38 + 8 * 256));
+  const double unc_bytes = (double)rows * 20, ref_bytes = (double)rows * 38;
+  const double u8 = wide_pass<256, 1, 8, false, 32, 32, 64, 8, 8, 8, 8>(wide, rows, 256, sink, reps);
+  const double u2 = wide_pass<256, 4, 2, true, 8, 8, 16, 2, 2, 2, 2>(wide, rows, 256, sink, reps);
+  const double r4 = wide_pass<256, 1, 4, false, 16, 32, 32, 32, 32, 4, 4>(wide, rows, 256, sink, reps);
+  const double r2 = wide_pass<256, 4, 2, true, 8, 16, 16, 16, 16, 2, 2>(wide, rows, 256, sink, reps);
+  CHECK(hipFree(wide));
+  const char* fmt = "{\"tile_rows\": %d, \"nontemporal\": %s, \"threads_per_block\": 256, \"tuples_per_thread\": 256, "
+                    "\"ms\": %.4f, \"gbps\": %.1f}";
+  std::printf("], \"uncompressed\": [");
+  std::printf(fmt, 8, "false", u8, unc_bytes / u8 / 1e6);
+  std::printf(", ");
+  std::printf(fmt, 2, "true", u2, unc_bytes / u2 / 1e6);
+  std::printf("], \"reference\": [");
+  std::printf(fmt, 4, "false", r4, ref_bytes / r4 / 1e6);
+  std::printf(", ");
+  std::printf(fmt, 2, "true", r2, ref_bytes / r2 / 1e6);
+  std::printf("], \"best_columns_gbps\": %.1f, \"best_streaming_gbps\": %.1f, \"best_flat_gbps\": %.1f, "
+              "\"best_uncompressed_gbps\": %.1f, \"best_reference_gbps\": %.1f}\n",
+              bytes / best_cols / 1e6, bytes / best_stream / 1e6, n16 * 16 / best_flat / 1e6,
+              unc_bytes / std::min(u8, u2) / 1e6, ref_bytes / std::min(r4, r2) / 1e6);
   CHECK(hipFree(sink));
   return 0;
 }

@@ -50,6 +50,10 @@ __device__ __forceinline__ void comp_or(uint4& v, int i, uint32_t bits) {
 __device__ __forceinline__ void comp_set(uint4& v, int i, uint32_t bits) {
   if (i == 0) v.x = bits; else if (i == 1) v.y = bits; else if (i == 2) v.z = bits; else v.w = bits;
 }
+__device__ __forceinline__ uint32_t comp(const uint2& v, int i) { return i == 0 ? v.x : v.y; }
+__device__ __forceinline__ void comp_set(uint2& v, int i, uint32_t bits) {
+  if (i == 0) v.x = bits; else v.y = bits;
+}
 __device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // Rows of the tile that lie inside [lo, hi): bit j set for row tile*TILE + j.
@@ -98,6 +102,7 @@ __device__ __forceinline__ uint2 nt_load(const uint2* p) {
   return make_uint2(v.x, v.y);
 }
 __device__ __forceinline__ uint32_t nt_load(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ uint32_t nt_load(const uint16_t* p) { return __builtin_nontemporal_load(p); }
 
 // Q1_LAYOUT_COMPRESSED / Q1_LAYOUT_COMPRESSED_PUSHDOWN: 16 rows per tile.
 template <bool PUSHDOWN>
@@ -384,6 +389,83 @@ struct UncompressedTile {
   }
 };
 
+// Streaming variant for the uncompressed layout (used with the shared placement): 2 rows
+// per tile, so that each load instruction of a wavefront reads one contiguous stretch of
+// its column with at most 16 bytes per lane: 64 lanes x 8 bytes of ship dates and
+// quantities, x 16 bytes of prices, x 2 bytes of discounts, taxes, return flags and line
+// statuses.  All loads are non-temporal (each byte is read once).
+struct UncompressedStreamTile {
+  static constexpr int TILE = 2;
+  static constexpr int GROUP = 4;  // tiles loaded together per lane
+  uint2 s, q;              // 2 x int32 ship date, quantity (hundredths)
+  uint4 p;                 // 2 x int64 price as (lo, hi) pairs
+  uint32_t d, x, rf, ls;   // 2 x int8 / char in the low half-word
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    s = nt_load(reinterpret_cast<const uint2*>(c.ship_date) + t);
+    q = nt_load(reinterpret_cast<const uint2*>(c.quantity) + t);
+    p = nt_load(reinterpret_cast<const uint4*>(c.extended_price) + t);
+    d = nt_load(reinterpret_cast<const uint16_t*>(c.discount) + t);
+    x = nt_load(reinterpret_cast<const uint16_t*>(c.tax) + t);
+    rf = nt_load(reinterpret_cast<const uint16_t*>(c.return_flag) + t);
+    ls = nt_load(reinterpret_cast<const uint16_t*>(c.line_status) + t);
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    s = q = make_uint2(0, 0);
+    p = make_uint4(0, 0, 0, 0);
+    d = x = rf = ls = 0;
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        const uint32_t sv = (uint32_t)reinterpret_cast<const int32_t*>(c.ship_date)[r];
+        const uint32_t qv = (uint32_t)reinterpret_cast<const int32_t*>(c.quantity)[r];
+        const uint64_t pv = (uint64_t)reinterpret_cast<const int64_t*>(c.extended_price)[r];
+        comp_set(s, j, sv);
+        comp_set(q, j, qv);
+        comp_set(p, 2 * j, (uint32_t)pv);
+        comp_set(p, 2 * j + 1, (uint32_t)(pv >> 32));
+        d |= (uint32_t)reinterpret_cast<const uint8_t*>(c.discount)[r] << (8 * j);
+        x |= (uint32_t)reinterpret_cast<const uint8_t*>(c.tax)[r] << (8 * j);
+        rf |= (uint32_t)reinterpret_cast<const uint8_t*>(c.return_flag)[r] << (8 * j);
+        ls |= (uint32_t)reinterpret_cast<const uint8_t*>(c.line_status)[r] << (8 * j);
+      }
+    }
+  }
+  __device__ __forceinline__ static uint32_t byte(uint32_t v, int j) { return (v >> (8 * j)) & 0xFFu; }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return encode_return_flag(byte(rf, j)) << 1 | encode_line_status(byte(ls, j));
+  }
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const { return (int32_t)comp(s, j) <= cut; }
+  // Every row inside the TPC-H value domain (discount, tax in [0, 15], quantity in
+  // [0, 6300] hundredths, price in [0, 2^24)); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    return max(q.x, q.y) <= 6300u && (p.y | p.w) == 0 && ((p.x | p.z) & 0xFF000000u) == 0 &&
+           ((d | x) & 0xF0F0u) == 0;
+  }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = comp(q, j);
+    r.disc = byte(d, j);
+    r.tax = byte(x, j);
+    r.price = comp(p, 2 * j);
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = (int32_t)comp(q, j);
+    r.disc = (int8_t)byte(d, j);
+    r.tax = (int8_t)byte(x, j);
+    r.price = (int64_t)((uint64_t)comp(p, 2 * j + 1) << 32 | comp(p, 2 * j));
+    return r;
+  }
+};
+
 // Q1_LAYOUT_REFERENCE: int32 ship, int64 qty/price/disc/tax, char rf/ls.
 struct ReferenceTile {
   static constexpr int TILE = 4;
@@ -450,6 +532,86 @@ struct ReferenceTile {
       dx |= d[k].x | d[k].z | x[k].x | x[k].z;
     }
     return hi == 0 && qmax <= 6300u && (plo & 0xFF000000u) == 0 && (dx & ~15u) == 0;
+  }
+  template <bool EDGE>
+  __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
+    FastRow r;
+    r.pass = EDGE ? ((valid >> j & 1) && pass(j, cut)) : pass(j, cut);
+    r.gid = gid(j);
+    r.qty = lo32(q, j);
+    r.disc = lo32(d, j);
+    r.tax = lo32(x, j);
+    r.price = lo32(p, j);
+    return r;
+  }
+  __device__ __forceinline__ WideRow wide(int j, int32_t cut, uint32_t valid) const {
+    WideRow r;
+    const bool ok = (valid >> j & 1) && pass(j, cut);
+    r.gid = ok ? gid(j) : kNoGroup;
+    r.qty = get64(q, j);
+    r.disc = get64(d, j);
+    r.tax = get64(x, j);
+    r.price = get64(p, j);
+    return r;
+  }
+};
+
+// Streaming variant for the reference layout (used with the shared placement): 2 rows per
+// tile, each load instruction of a wavefront one contiguous stretch of its column with at
+// most 16 bytes per lane (64 lanes x 8 bytes of ship dates, x 16 bytes of each int64
+// column, x 2 bytes of return flags and line statuses), all loads non-temporal.
+struct ReferenceStreamTile {
+  static constexpr int TILE = 2;
+  static constexpr int GROUP = 4;  // tiles loaded together per lane
+  uint2 s;                 // 2 x int32 ship date
+  uint4 q, p, d, x;        // 2 x int64 as (lo, hi) pairs
+  uint32_t rf, ls;         // 2 x char in the low half-word
+
+  __device__ __forceinline__ void load(const q1_columns& c, uint64_t t) {
+    s = nt_load(reinterpret_cast<const uint2*>(c.ship_date) + t);
+    q = nt_load(reinterpret_cast<const uint4*>(c.quantity) + t);
+    p = nt_load(reinterpret_cast<const uint4*>(c.extended_price) + t);
+    d = nt_load(reinterpret_cast<const uint4*>(c.discount) + t);
+    x = nt_load(reinterpret_cast<const uint4*>(c.tax) + t);
+    rf = nt_load(reinterpret_cast<const uint16_t*>(c.return_flag) + t);
+    ls = nt_load(reinterpret_cast<const uint16_t*>(c.line_status) + t);
+  }
+  __device__ __forceinline__ static void put64(uint4& v, int j, uint64_t val) {
+    comp_set(v, 2 * j, (uint32_t)val);
+    comp_set(v, 2 * j + 1, (uint32_t)(val >> 32));
+  }
+  __device__ __forceinline__ void load_edge(const q1_columns& c, uint64_t t, uint32_t valid) {
+    s = make_uint2(0, 0);
+    q = p = d = x = make_uint4(0, 0, 0, 0);
+    rf = ls = 0;
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (valid >> j & 1) {
+        const uint64_t r = t * TILE + j;
+        comp_set(s, j, (uint32_t)reinterpret_cast<const int32_t*>(c.ship_date)[r]);
+        put64(q, j, (uint64_t)reinterpret_cast<const int64_t*>(c.quantity)[r]);
+        put64(p, j, (uint64_t)reinterpret_cast<const int64_t*>(c.extended_price)[r]);
+        put64(d, j, (uint64_t)reinterpret_cast<const int64_t*>(c.discount)[r]);
+        put64(x, j, (uint64_t)reinterpret_cast<const int64_t*>(c.tax)[r]);
+        rf |= (uint32_t)reinterpret_cast<const uint8_t*>(c.return_flag)[r] << (8 * j);
+        ls |= (uint32_t)reinterpret_cast<const uint8_t*>(c.line_status)[r] << (8 * j);
+      }
+    }
+  }
+  __device__ __forceinline__ static uint32_t lo32(const uint4& v, int j) { return comp(v, 2 * j); }
+  __device__ __forceinline__ static int64_t get64(const uint4& v, int j) {
+    return (int64_t)((uint64_t)comp(v, 2 * j + 1) << 32 | comp(v, 2 * j));
+  }
+  __device__ __forceinline__ uint32_t gid(int j) const {
+    return encode_return_flag((rf >> (8 * j)) & 0xFFu) << 1 | encode_line_status((ls >> (8 * j)) & 0xFFu);
+  }
+  __device__ __forceinline__ bool pass(int j, int32_t cut) const { return (int32_t)comp(s, j) <= cut; }
+  // Every row inside the TPC-H value domain (int64 discount, tax in [0, 15], quantity in
+  // [0, 6300], price in [0, 2^24)); rows zeroed by load_edge pass.
+  __device__ __forceinline__ bool in_domain() const {
+    const uint32_t hi = q.y | q.w | p.y | p.w | d.y | d.w | x.y | x.w;
+    return hi == 0 && max(q.x, q.z) <= 6300u && ((p.x | p.z) & 0xFF000000u) == 0 &&
+           ((d.x | d.z | x.x | x.z) & ~15u) == 0;
   }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
@@ -817,6 +979,10 @@ template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN, Q1_PLACE_PER_THREAD
   using T = CompressedStreamTile<true>;
 };
 template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN, kPlaceSharedNarrow> { using T = CompressedStreamTile<true>; };
+template <> struct ScanTileOf<Q1_LAYOUT_UNCOMPRESSED, Q1_PLACE_PER_THREAD_SHARED_MEM> { using T = UncompressedStreamTile; };
+template <> struct ScanTileOf<Q1_LAYOUT_UNCOMPRESSED, kPlaceSharedNarrow> { using T = UncompressedStreamTile; };
+template <> struct ScanTileOf<Q1_LAYOUT_REFERENCE, Q1_PLACE_PER_THREAD_SHARED_MEM> { using T = ReferenceStreamTile; };
+template <> struct ScanTileOf<Q1_LAYOUT_REFERENCE, kPlaceSharedNarrow> { using T = ReferenceStreamTile; };
 
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
@@ -1309,6 +1475,8 @@ __global__ void build_filter(const uint16_t* ship, uint64_t rows, int32_t cut, u
 template <int L, int P>
 __global__ void __launch_bounds__(kMaxBlock)
 batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
+  // 16-row tiles with ordinary loads: the small streaming tiles of the scan kernel's LDS
+  // placement measured slower here, where each block's share spans several requests.
   using Tile = typename TileOf<L>::T;
   using Acc = typename AccOf<P, kBatchBlock>::A;
   constexpr int TILE = Tile::TILE;
