@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
     ap.add_argument("--no-layouts", dest="layouts", action="store_false",
                     help="skip the pushdown / uncompressed / reference layouts leg")
+    ap.add_argument("--no-host", dest="host", action="store_false",
+                    help="skip the host-resident streaming and CPU/GPU split leg")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="length of each offered rate in the serving leg (0 skips it)")
     ap.add_argument("--serve-rates", default="2000,8000", help="offered request rates (req/s), comma-separated")
@@ -85,8 +87,10 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
     """The oracle's restatements of the reference CPU kernels on a bounded sample of the
     same generator's rows (reference layout; ComprData for X100): KernelNaive
     (cpu/kernel_naive.hpp) morsel-parallel on `threads` threads (the reported value) and on
-    one thread, and KernelX100 (cpu/kernel_x100.hpp, 1024-row vectors) on both.  Each is
-    the mean of the hot passes after one cold pass (cpu/q1.cpp:268-299)."""
+    one thread, KernelHyPer<true> (cpu/kernel_hyper.hpp) on one thread, and KernelX100
+    (cpu/kernel_x100.hpp, 1024-row vectors) on both.  Each is the mean of the hot passes
+    after one cold pass (cpu/q1.cpp:268-299).  Also returns the oracle's groups for these
+    rows, the checker of the device results on the same rows."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle_lib import Oracle
 
@@ -108,15 +112,17 @@ def cpu_baseline(q1, rows: int, threads: int, passes: int):
 
     naive_mt = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, threads))
     naive_1 = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, 1))
+    hyper_1 = timed(lambda: o.hyper(cols, 0, rows, 729999))
     x100_mt = timed(lambda: o.x100(comp, 0, rows, 729999, threads))
     x100_1 = timed(lambda: o.x100(comp, 0, rows, 729999, 1))
+    want, _ = o.naive(cols, 0, rows, 729999)
     return {"value": naive_mt, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"{rows} rows, reference layout (38 B/row), oracle KernelNaive restatement, "
                       f"morsel-parallel on {threads} threads, mean of {passes - 1} hot passes",
             "ms_per_pass": rows / naive_mt * 1e3,
-            "naive_1_thread_rows_s": naive_1,
+            "naive_1_thread_rows_s": naive_1, "hyper_1_thread_rows_s": hyper_1,
             "x100_rows_s": x100_mt, "x100_1_thread_rows_s": x100_1,
-            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}, want
 
 
 def hbm_probe(rows: int):
@@ -209,6 +215,99 @@ def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5, probe=
         t.free()
     out["exact_vs_compressed"] = exact
     return out
+
+
+def as_dicts(groups):
+    return [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
+                 sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in groups]
+
+
+def uncompressed_leg(q1, ctx, rows: int, seed: int, reps: int = 20):
+    """`rows` device-resident uncompressed rows (20 B/row: int32 ship date and quantity,
+    int64 price, int8 discount, tax, flags) through every placement: kernel time of q1_run
+    with the placement's tuned launch, mean of `reps` runs after one warm-up.  The same
+    seed and rows as the CPU baseline's sample, whose oracle result checks every placement."""
+    t = ctx.generate(q1.UNCOMPRESSED, seed, rows)
+    algo = rows * q1.BITS_PER_ROW[q1.UNCOMPRESSED] // 8
+    out, results = {"rows": rows, "bytes_per_row": 20.0, "placements": {}}, []
+    try:
+        for p in (q1.IN_REGISTERS, q1.IN_REGISTERS_PER_THREAD, q1.LOCAL_MEM, q1.PER_THREAD_SHARED_MEM, q1.GLOBAL):
+            name = q1.PLACEMENT_NAMES[p]
+            got = ctx.run_raw(t, 0, rows, 729999, p)
+            results.append(q1.result_to_list(got))
+            ms = []
+            for _ in range(reps):
+                ctx.run_raw(t, 0, rows, 729999, p)
+                ms.append(ctx.last_timing().kernel_ms)
+            tm = ctx.last_timing()
+            ms = sum(ms) / len(ms)
+            out["placements"][name] = {"kernel_ms": ms, "rows_s": rows / (ms * 1e-3),
+                                       "gbps": algo / (ms * 1e-3) / 1e9,
+                                       "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                       "threads_per_block": tm.threads_per_block,
+                                       "tuples_per_thread": tm.tuples_per_thread}
+    finally:
+        t.free()
+    out["placements_agree"] = all(r == results[0] for r in results)
+    out["_results"] = results
+    return out
+
+
+def host_leg(q1, ctx, table, rows: int, seed: int, want, cpu_threads: int, reps: int = 3):
+    """Host-resident input, the reference's own mode: the same compressed rows generated
+    into pinned host memory.  (1) q1_run_streaming: chunks of 16M rows copied over 4
+    streams with the copies overlapping the kernels (src/main.cu:567-724), end-to-end time
+    per query.  (2) q1_run_split, the CPU/GPU coprocessing split (src/main.cu:513-550):
+    the device scans [0, split) of the resident table while `cpu_threads` host threads
+    aggregate [split, rows) of the host copy, at the reference's default CPU fraction 0.5
+    and at the fraction that balances the two sides' measured rates.  Every result is
+    checked against the device-resident one."""
+    cols = ctx.pinned_columns(q1.COMPRESSED, rows)
+    try:
+        t0 = time.perf_counter()
+        q1.fill_host(q1.COMPRESSED, seed, cols, rows, 0, cpu_threads)
+        gen_s = time.perf_counter() - t0
+        nbytes = rows * q1.BITS_PER_ROW[q1.COMPRESSED] // 8
+        chunk, streams = 1 << 24, 4
+        dom = q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN)   # generated rows lie in the TPC-H domain
+        exact = ctx.run_streaming(q1.COMPRESSED, cols, rows, 729999, q1.AUTO, dom, chunk, streams) == want
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            got = ctx.run_streaming(q1.COMPRESSED, cols, rows, 729999, q1.AUTO, dom, chunk, streams)
+            walls.append(time.perf_counter() - t0)
+            exact = exact and got == want
+        wall = sum(walls) / len(walls)
+        out = {"rows": rows, "host_generate_s": gen_s,
+               "streaming": {"chunk_rows": chunk, "streams": streams, "ms_per_query": wall * 1e3,
+                             "rows_s": rows / wall, "h2d_gbps": nbytes / wall / 1e9, "exact": exact}}
+        align = 1 << 21
+        split_out = []
+        gpu_rate = cpu_rate = None
+        for frac in (0.5, None):
+            if frac is None:   # balance the sides with the rates the 0.5 split measured
+                frac = cpu_rate / (cpu_rate + gpu_rate)
+            split = rows - int(rows * frac)
+            split -= split % align
+            got, cpu_ms, gpu_ms = ctx.run_split(table, cols, rows, frac, 729999, q1.AUTO, None, align, cpu_threads)
+            ok = got == want
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                got, cpu_ms, gpu_ms = ctx.run_split(table, cols, rows, frac, 729999, q1.AUTO, None, align,
+                                                    cpu_threads)
+                ok = ok and got == want
+            wall = (time.perf_counter() - t0) / reps
+            gpu_rate = split / (gpu_ms * 1e-3) if gpu_ms > 0 else gpu_rate
+            cpu_rate = (rows - split) / (cpu_ms * 1e-3) if cpu_ms > 0 and rows > split else cpu_rate
+            split_out.append({"cpu_fraction": frac, "gpu_rows": split, "cpu_rows": rows - split,
+                              "gpu_ms": gpu_ms, "cpu_ms": cpu_ms, "wall_ms": wall * 1e3,
+                              "gpu_rows_s": split / (gpu_ms * 1e-3) if gpu_ms > 0 else None,
+                              "cpu_rows_s": (rows - split) / (cpu_ms * 1e-3) if cpu_ms > 0 else None,
+                              "cpu_threads": cpu_threads, "exact": ok})
+        out["split"] = split_out
+        return out
+    finally:
+        ctx.free_pinned_columns(cols)
 
 
 def serving_leg(q1, ctx, seconds: float, rates):
@@ -370,7 +469,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": kernel_ms, "algorithmic_bytes": algo_bytes,
                      "read_ceiling_gbps": ceiling,
-                     "frac_of_read_ceiling": achieved / ceiling if ceiling else None},
+                     "frac_of_read_ceiling": achieved / ceiling if ceiling else None,
+                     "copy_peak_gbps": probe.get("copy_gbps") if probe else None},
     }
     if args.layouts and layout == q1.COMPRESSED:
         log("bench: layouts leg")
@@ -407,6 +507,14 @@ def main():
             "frac": algo_bytes / (wms * 1e-3) / 1e9 / HBM_PEAK_GBS, "threads_per_block": 256,
             "tuples_per_thread": 1024, "exact_vs_fast": exact}
         log(f"bench: layouts {out['layouts']}")
+    if args.layouts and layout == q1.COMPRESSED:
+        log(f"bench: {args.cpu_rows} uncompressed rows through every placement")
+        out["uncompressed_60M"] = uncompressed_leg(q1, ctx, args.cpu_rows, 0x51C00000)
+        log(f"bench: uncompressed_60M {out['uncompressed_60M']['placements']}")
+    if args.host and layout == q1.COMPRESSED:
+        log("bench: host-resident leg (streaming, CPU/GPU split)")
+        out["host_resident"] = host_leg(q1, ctx, table, rows, 0x51C00002, one, args.cpu_threads)
+        log(f"bench: host_resident {out['host_resident']}")
     if args.batch_requests:
         log(f"bench: batch leg, {args.batch_requests} requests")
         out["batch"] = batch_leg(q1, ctx, args.batch_requests)
@@ -417,7 +525,12 @@ def main():
         out["serving"] = serving_leg(q1, ctx, args.serve_seconds, rates)
     if not args.no_cpu:
         log(f"bench: CPU baseline on {args.cpu_rows} rows x {args.cpu_passes} passes, {args.cpu_threads} threads")
-        out["cpu_baseline"] = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)
+        out["cpu_baseline"], want = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)
+        if "uncompressed_60M" in out and out["uncompressed_60M"]["rows"] == args.cpu_rows:
+            out["uncompressed_60M"]["exact_vs_oracle"] = all(
+                as_dicts(r) == want for r in out["uncompressed_60M"].pop("_results"))
+    if "uncompressed_60M" in out:
+        out["uncompressed_60M"].pop("_results", None)
     else:
         out["cpu_baseline"] = None
     table.free()

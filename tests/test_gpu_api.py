@@ -213,3 +213,28 @@ def test_batch_every_layout_and_batch_placement(layout):
                     assert q1.result_to_list(outs[i]) == c.run(t, b, s, cut, q1.IN_REGISTERS), (p, n, i)
         for t in tables:
             t.free()
+
+
+def test_pinned_columns_stream_and_split(ctx):
+    """Pinned host columns filled in place by the host generator: streamed to the device and
+    split between the device table and host threads, both equal the device-resident run."""
+    rows = 1_000_003
+    t = ctx.generate(q1.COMPRESSED, SEED + 21, rows)
+    want = ctx.run(t, 0, rows, 729999)
+    cols = ctx.pinned_columns(q1.COMPRESSED, rows)
+    try:
+        assert set(cols) == {"ship_date", "quantity", "extended_price", "discount", "tax", "return_flag",
+                             "line_status"}
+        q1.fill_host(q1.COMPRESSED, SEED + 21, cols, rows, 0, 4)
+        note("case pinned streaming")
+        got = ctx.run_streaming(q1.COMPRESSED, cols, rows, 729999, q1.AUTO,
+                                q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN), 1 << 18, 3)
+        assert got == want
+        for frac in (0.0, 0.003, 0.5, 1.0):
+            note(f"case pinned split {frac}")
+            got, cpu_ms, gpu_ms = ctx.run_split(t, cols, rows, frac, 729999, q1.AUTO, None, 1 << 16, 4)
+            assert got == want, frac
+    finally:
+        ctx.free_pinned_columns(cols)
+        t.free()
+    assert cols == {}

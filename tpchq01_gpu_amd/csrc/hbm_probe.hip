@@ -6,8 +6,9 @@
 // aggregation (each lane only XORs what it loads), and then the same number of bytes as
 // one flat buffer, and with the streaming pattern of the shared placement (4-row tiles,
 // non-temporal loads); then the columns of the uncompressed (20 B/row) and reference
-// (38 B/row) layouts with the patterns of their kernels.  Each shape is timed as 20
-// back-to-back launches, as bench.py times the scan.  The fastest of these is the rate the scan could reach if its arithmetic
+// (38 B/row) layouts with the patterns of their kernels, and a plain 2 GiB device copy
+// (read + write rate).  Each shape is timed as 20 back-to-back launches, as bench.py
+// times the scan.  The fastest of these is the rate the scan could reach if its arithmetic
 // were free; the scan's own rate is judged against it and against the published 8 TB/s.
 //
 //   hbm_probe [rows]     (default 600000000; prints one JSON line)
@@ -83,6 +84,15 @@ __global__ __launch_bounds__(B) void read_columns_nt(Columns c, uint64_t tiles4,
     acc += s.x ^ s.y ^ q ^ d ^ x ^ p.x ^ p.y ^ p.z ^ p.w ^ rf[t] ^ (ls[t >> 1] >> (4 * (t & 1)));
   }
   if (acc == kSentinel) sink[0] = acc;
+}
+
+// A plain device copy, 16-byte loads and stores, each block a contiguous range: the
+// read-plus-write rate a stream copy reaches on this GPU.
+template <int B>
+__global__ __launch_bounds__(B) void copy_flat(const uint4* src, uint4* dst, uint64_t n, uint32_t per_thread) {
+  const uint64_t begin = (uint64_t)blockIdx.x * B * per_thread;
+  const uint64_t end = begin + (uint64_t)B * per_thread < n ? begin + (uint64_t)B * per_thread : n;
+  for (uint64_t i = begin + threadIdx.x; i < end; i += B) dst[i] = src[i];
 }
 
 // The wide layouts' columns: lane tiles of TILE rows, column k holding W[k] bytes per
@@ -305,6 +315,19 @@ int main(int argc, char** argv) {
   const double r4 = wide_pass<256, 1, 4, false, 16, 32, 32, 32, 32, 4, 4>(wide, rows, 256, sink, reps);
   const double r2 = wide_pass<256, 4, 2, true, 8, 16, 16, 16, 16, 2, 2>(wide, rows, 256, sink, reps);
   CHECK(hipFree(wide));
+
+  // Stream copy of 2 GiB into another 2 GiB: bytes read plus bytes written per second.
+  const uint64_t copy_bytes = 2ull << 30, n16c = copy_bytes / 16;
+  uint4 *csrc = nullptr, *cdst = nullptr;
+  CHECK(hipMalloc(&csrc, copy_bytes));
+  CHECK(hipMalloc(&cdst, copy_bytes));
+  CHECK(hipMemset(csrc, 0x5A, copy_bytes));
+  const uint32_t cpt = 64;
+  const uint64_t cblocks = (n16c + 256ull * cpt - 1) / (256ull * cpt);
+  const double copy_ms =
+      back_to_back_ms([&] { copy_flat<256><<<dim3((uint32_t)cblocks), dim3(256)>>>(csrc, cdst, n16c, cpt); }, reps);
+  CHECK(hipFree(csrc));
+  CHECK(hipFree(cdst));
   const char* fmt = "{\"tile_rows\": %d, \"nontemporal\": %s, \"threads_per_block\": 256, \"tuples_per_thread\": 256, "
                     "\"ms\": %.4f, \"gbps\": %.1f}";
   std::printf("], \"uncompressed\": [");
@@ -315,10 +338,12 @@ int main(int argc, char** argv) {
   std::printf(fmt, 4, "false", r4, ref_bytes / r4 / 1e6);
   std::printf(", ");
   std::printf(fmt, 2, "true", r2, ref_bytes / r2 / 1e6);
-  std::printf("], \"best_columns_gbps\": %.1f, \"best_streaming_gbps\": %.1f, \"best_flat_gbps\": %.1f, "
-              "\"best_uncompressed_gbps\": %.1f, \"best_reference_gbps\": %.1f}\n",
+  std::printf("], \"copy\": {\"bytes\": %llu, \"ms\": %.4f, \"gbps\": %.1f}", (unsigned long long)copy_bytes,
+              copy_ms, 2.0 * copy_bytes / copy_ms / 1e6);
+  std::printf(", \"best_columns_gbps\": %.1f, \"best_streaming_gbps\": %.1f, \"best_flat_gbps\": %.1f, "
+              "\"best_uncompressed_gbps\": %.1f, \"best_reference_gbps\": %.1f, \"copy_gbps\": %.1f}\n",
               bytes / best_cols / 1e6, bytes / best_stream / 1e6, n16 * 16 / best_flat / 1e6,
-              unc_bytes / std::min(u8, u2) / 1e6, ref_bytes / std::min(r4, r2) / 1e6);
+              unc_bytes / std::min(u8, u2) / 1e6, ref_bytes / std::min(r4, r2) / 1e6, 2.0 * copy_bytes / copy_ms / 1e6);
   CHECK(hipFree(sink));
   return 0;
 }

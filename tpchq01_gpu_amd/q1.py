@@ -254,7 +254,12 @@ def columns_struct(cols: dict) -> Columns:
 
 def generate_host(layout: int, seed: int, rows: int, row_begin: int = 0, threads: int = 0) -> dict:
     """Host columns of rows [row_begin, row_begin+rows) of the seeded generator."""
-    cols = empty_columns(layout, rows)
+    return fill_host(layout, seed, empty_columns(layout, rows), rows, row_begin, threads)
+
+
+def fill_host(layout: int, seed: int, cols: dict, rows: int, row_begin: int = 0, threads: int = 0) -> dict:
+    """Writes rows [row_begin, row_begin+rows) of the seeded generator into existing host
+    columns of the layout's shapes (e.g. pinned ones from Context.pinned_columns)."""
     st = columns_struct(cols)
     _check(lib.q1_generate_host(layout, seed, row_begin, rows, C.byref(st), threads), "q1_generate_host")
     return cols
@@ -474,3 +479,25 @@ class Context:
 
     def host_free(self, p: int):
         lib.q1_host_free(self.handle, p)
+
+    def pinned_columns(self, layout: int, rows: int) -> dict:
+        """Host numpy columns of the layout's shapes in pinned memory (q1_host_alloc), for
+        q1_run_streaming and q1_run_split; release them with free_pinned_columns."""
+        sizes = column_bytes(layout, rows)
+        cols = {}
+        try:
+            for i, name in enumerate(COLUMN_NAMES):
+                dt = _DTYPES[layout][i]
+                if dt is None or sizes[i] == 0:
+                    continue
+                buf = (C.c_uint8 * sizes[i]).from_address(self.host_alloc(sizes[i]))
+                cols[name] = np.frombuffer(buf, dtype=dt)
+        except Exception:
+            self.free_pinned_columns(cols)
+            raise
+        return cols
+
+    def free_pinned_columns(self, cols: dict):
+        for arr in cols.values():
+            self.host_free(arr.ctypes.data)
+        cols.clear()
