@@ -238,3 +238,23 @@ def test_pinned_columns_stream_and_split(ctx):
         ctx.free_pinned_columns(cols)
         t.free()
     assert cols == {}
+
+
+def test_batch_small_grids_span_many_requests(ctx):
+    """Grids of 1, 2 and 3 blocks give every block a share that crosses many request
+    boundaries (256 requests of about 65 tiles, aligned and unaligned starts): every request
+    must still be counted whole, as with the default grid."""
+    t = ctx.generate(q1.COMPRESSED, SEED + 31, 70_000)
+    for rows, begins in ((1040, [0] * 256), (1040, [16 * (i % 7) for i in range(256)]),
+                         (1037, [3 + (i % 5) for i in range(256)])):
+        reqs = [q1.Request(t.handle, b, rows, 729999 - (i % 3), 0) for i, b in enumerate(begins)]
+        n = len(reqs)
+        want = [ctx.run(t, b, rows, 729999 - (i % 3), q1.IN_REGISTERS) for i, b in enumerate(begins)]
+        for blocks in (1, 2, 3, 0):
+            outs = (q1.Result * n)()
+            note(f"case batch search rows={rows} blocks={blocks}")
+            ctx.check(q1.lib.q1_run_batch(ctx.handle, (q1.Request * n)(*reqs), n,
+                                          C.byref(q1.make_launch(0, 0, blocks)), outs), "q1_run_batch")
+            for i in range(n):
+                assert q1.result_to_list(outs[i]) == want[i], (rows, blocks, i)
+    t.free()
