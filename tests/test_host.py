@@ -171,3 +171,65 @@ def test_shipped_tuning_table_is_well_formed():
     assert set(best) == set(layouts)
     for l in layouts:
         assert (l, best[l]) in launches
+
+
+def _compressed_cols(ship, qty, price, disc, tax, rf, ls):
+    """Compressed host columns from per-row arrays (rf codes 0..3, ls bits)."""
+    n = len(ship)
+    cols = q1.empty_columns(q1.COMPRESSED, n)
+    cols["ship_date"][:] = ship
+    cols["quantity"][:] = qty
+    cols["extended_price"][:] = price
+    cols["discount"][:] = disc
+    cols["tax"][:] = tax
+    for i in range(n):
+        cols["return_flag"][i // 16] |= np.uint32(int(rf[i]) << (2 * (i % 16)))
+        cols["line_status"][i // 32] |= np.uint32(int(ls[i]) << (i % 32))
+    return cols
+
+
+def test_cpu_coprocessor_compressed_extremes_cross_int64():
+    """Compressed columns at their type limits (u8 discount, tax, quantity 255, u32 price
+    2^32-1) in one group over 40,000 rows: the charge sum passes -2^63, so the int64 runs of
+    2^14 rows must widen exactly.  Every row is the same, so each sum is the row count times
+    the row's value."""
+    n = 40_000
+    z = np.zeros(n, dtype=np.int64)
+    cols = _compressed_cols(z, z + 255, z + (2**32 - 1), z + 255, z + 255, z, z)
+    g = q1.cpu_run(q1.COMPRESSED, cols, 0, n, 727563, threads=3)[0]
+    disc_price = (100 - 255) * (2**32 - 1)
+    assert g.count == n
+    assert g.sum_qty == n * 255 * 100
+    assert g.sum_base_price == n * (2**32 - 1)
+    assert g.sum_disc == n * 255
+    assert g.sum_disc_price == n * disc_price
+    assert g.sum_charge == n * disc_price * (255 + 100)
+    assert g.sum_charge < -(2**63)
+
+
+def test_cpu_coprocessor_compressed_full_range_random():
+    """Random values over the compressed columns' whole ranges (rf codes 0..3, so some rows
+    fall outside the six groups), 50,000 rows, against Python integer arithmetic."""
+    rng = np.random.default_rng(5)
+    n = 50_000
+    ship = rng.integers(0, 2**16, n)
+    qty, disc, tax = (rng.integers(0, 256, n) for _ in range(3))
+    price = rng.integers(0, 2**32, n)
+    rf, ls = rng.integers(0, 4, n), rng.integers(0, 2, n)
+    cols = _compressed_cols(ship, qty, price, disc, tax, rf, ls)
+    cut = 40_000
+    got = q1.cpu_run(q1.COMPRESSED, cols, 0, n, 727563 + cut, threads=4)
+    want = [dict(sum_qty=0, sum_base_price=0, sum_disc_price=0, sum_charge=0, sum_disc=0, count=0) for _ in range(6)]
+    for i in range(n):
+        slot = int(rf[i]) << 1 | int(ls[i])
+        if int(ship[i]) > cut or slot >= 6:
+            continue
+        w = want[slot]
+        dp = (100 - int(disc[i])) * int(price[i])
+        w["sum_qty"] += int(qty[i]) * 100
+        w["sum_base_price"] += int(price[i])
+        w["sum_disc"] += int(disc[i])
+        w["sum_disc_price"] += dp
+        w["sum_charge"] += dp * (100 + int(tax[i]))
+        w["count"] += 1
+    assert as_dicts(got) == want

@@ -35,7 +35,7 @@ void parallel_ranges(uint64_t n, uint64_t align, int threads, F&& fn) {
   for (auto& t : ts) t.join();
 }
 
-bool compressed_family(q1_layout l) { return l == Q1_LAYOUT_COMPRESSED || l == Q1_LAYOUT_COMPRESSED_PUSHDOWN; }
+constexpr bool compressed_family(q1_layout l) { return l == Q1_LAYOUT_COMPRESSED || l == Q1_LAYOUT_COMPRESSED_PUSHDOWN; }
 
 // ---- row access per layout --------------------------------------------------------------
 struct RowVals {
@@ -75,7 +75,9 @@ inline RowVals read_row(const q1_columns& c, uint64_t i) {
   return r;
 }
 
-struct Table6 {
+// One worker's private table; cache-line aligned so that neighbouring workers' tables
+// share no line.
+struct alignas(64) Table6 {
   uint64_t qty[6], price[6], disc[6], cnt[6];
   i128 dp[6], ch[6];
   void clear() { std::memset(this, 0, sizeof(*this)); }
@@ -87,8 +89,25 @@ struct Table6 {
   }
 };
 
+// Compressed columns hold u8 discount and tax and a u32 price, so a row's disc_price is
+// below 2^40 and its charge below 2^48 in magnitude, and 2^14 rows of them sum exactly in
+// int64: those layouts add the two products in int64 per run of kRun rows and widen to
+// 128 bits once per run.  The wide layouts' int64 columns keep KernelNaive's wrapping
+// int64 products with 128-bit sums per row.
+constexpr uint64_t kRun = 1u << 14;
+
 template <int L>
-void aggregate_range(const q1_columns& c, uint64_t b, uint64_t e, int64_t cut, Table6& t) {
+void aggregate_range(const q1_columns& c, uint64_t b, uint64_t e, int64_t cut, Table6& out) {
+  if (compressed_family((q1_layout)L) && e - b > kRun) {
+    for (uint64_t r = b; r < e; r += kRun) aggregate_range<L>(c, r, r + kRun < e ? r + kRun : e, cut, out);
+    return;
+  }
+  // A table local to the morsel: the column pointers (char / uint8 columns may alias any
+  // object) cannot point into it, so its sums stay out of the loads and stores of the
+  // worker's shared table until the morsel ends.
+  Table6 t;
+  t.clear();
+  int64_t dp64[6] = {}, ch64[6] = {};
   for (uint64_t i = b; i < e; i++) {
     bool pass;
     if (L == Q1_LAYOUT_COMPRESSED_PUSHDOWN)
@@ -107,9 +126,21 @@ void aggregate_range(const q1_columns& c, uint64_t b, uint64_t e, int64_t cut, T
     t.price[g] += (uint64_t)r.price;
     t.disc[g] += (uint64_t)r.disc;
     t.cnt[g] += 1;
-    t.dp[g] += (i128)disc_price;
-    t.ch[g] += (i128)charge;
+    if (compressed_family((q1_layout)L)) {
+      dp64[g] += disc_price;
+      ch64[g] += charge;
+    } else {
+      t.dp[g] += (i128)disc_price;
+      t.ch[g] += (i128)charge;
+    }
   }
+  if (compressed_family((q1_layout)L)) {
+    for (int g = 0; g < 6; g++) {
+      t.dp[g] += (i128)dp64[g];
+      t.ch[g] += (i128)ch64[g];
+    }
+  }
+  out.add(t);
 }
 
 using RangeFn = void (*)(const q1_columns&, uint64_t, uint64_t, int64_t, Table6&);
