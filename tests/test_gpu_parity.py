@@ -320,3 +320,31 @@ def test_streaming_matches_oracle(ctx, oracle):
     # without the domain flag the 64-bit kernels run, same answer
     got = ctx.run_streaming(q1.COMPRESSED, h[q1.COMPRESSED], rows, 729906, q1.IN_REGISTERS, None, 1 << 20, 2)
     assert as_dicts(got) == want
+
+
+@pytest.mark.parametrize("pattern", ["every_7th", "one_stretch"])
+def test_unused_return_flag_code_never_counts(ctx, oracle, pattern):
+    """The compressed return-flag column has a fourth 2-bit code (3) that no real row has;
+    rows carrying it fall outside the six groups and are skipped.  Rows with it spread over
+    the table (most tile groups hold one) or in one 64-row stretch (most tile groups hold
+    none, so they take the kernels' path for groups known to be clean): every placement
+    matches the oracle's compressed kernel."""
+    n = 300_017
+    clean = q1.generate_host(q1.COMPRESSED, SEED + 9, n)
+    cols = {k: v.copy() for k, v in clean.items()}
+    rows = np.arange(n)
+    bad = rows[rows % 7 == 3] if pattern == "every_7th" else rows[100_000:100_064]
+    for i in bad:
+        cols["return_flag"][i // 16] |= np.uint32(3 << (2 * (i % 16)))
+    cutoff = 729999
+    want = [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 0, n, cutoff)]
+    # the planted rows passed the filter and no longer count
+    assert sum(g["count"] for g in want) < sum(g["count"] for g in oracle.compressed_kernel(clean, 0, n, cutoff))
+    t = ctx.upload(q1.COMPRESSED, cols, n)
+    for p in PLACEMENTS:
+        for launch in (None, q1.make_launch(64, 16), q1.make_launch(256, 256)):
+            note(f"case invalid rf {pattern} placement={p} launch={launch and (launch.threads_per_block, launch.tuples_per_thread)}")
+            assert as_dicts(ctx.run(t, 0, n, cutoff, p, launch)) == want
+    assert as_dicts(ctx.run(t, 5, n - 9, cutoff, q1.PER_THREAD_SHARED_MEM)) == \
+        [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 5, n - 4, cutoff)]
+    t.free()

@@ -179,6 +179,30 @@ struct CompressedTile {
   __device__ __forceinline__ uint32_t byte(const uint4& v, int j) const {
     return (comp(v, j >> 2) >> (8 * (j & 3))) & 0xFFu;
   }
+  // Rows carrying the unused return-flag code 3 (slot 6 or 7): bit 2j set for row j.
+  __device__ __forceinline__ uint32_t bad_codes() const { return rf & (rf >> 1) & 0x55555555u; }
+  // Turns those rows into rows that fail the filter (ship date 0xFFFF, above every
+  // cutoff below 65535; pushdown: filter bit cleared) with code 0, so every row's slot is
+  // below 6.
+  __device__ void drop_bad_codes() {
+    const uint32_t inv = bad_codes();
+    rf &= ~(inv * 3u);
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      if (inv >> (2 * j) & 1) {
+        if (PUSHDOWN)
+          flt &= ~(1u << j);
+        else
+          comp_or(sd[j >> 3], (j >> 1) & 3, 0xFFFFu << (16 * (j & 1)));
+      }
+    }
+  }
+  template <int U>
+  __device__ __forceinline__ static void load_group(CompressedTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+#pragma unroll
+    for (int m = 0; m < U; m++) g[m].load(c, t0 + (uint64_t)m * stride);
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -277,6 +301,43 @@ struct CompressedStreamTile {
     return __builtin_amdgcn_ubfe(rf, 2 * j, 2) << 1 | __builtin_amdgcn_ubfe(ls, j, 1);
   }
   __device__ __forceinline__ static uint32_t byte(uint32_t v, int j) { return (v >> (8 * j)) & 0xFFu; }
+  // Rows carrying the unused return-flag code 3 (slot 6 or 7): bit 2j set for row j.
+  __device__ __forceinline__ uint32_t bad_codes() const { return rf & (rf >> 1) & 0x55u; }
+  // Turns those rows into rows that fail the filter (ship date 0xFFFF, above every
+  // cutoff below 65535; pushdown: filter bit cleared) with code 0, so every row's slot is
+  // below 6.
+  __device__ void drop_bad_codes() {
+    const uint32_t inv = bad_codes();
+    rf &= ~(inv * 3u);
+    if (PUSHDOWN) {
+      flt &= ~((inv & 1u) | (inv >> 1 & 2u) | (inv >> 2 & 4u) | (inv >> 3 & 8u));
+    } else {
+      sd.x |= (inv & 1u ? 0xFFFFu : 0u) | (inv & 4u ? 0xFFFF0000u : 0u);
+      sd.y |= (inv & 0x10u ? 0xFFFFu : 0u) | (inv & 0x40u ? 0xFFFF0000u : 0u);
+    }
+  }
+  // U tiles a block-stride apart (stride even): their packed line-status / filter nibbles
+  // sit stride / 2 bytes apart from one base and at the same half of their byte.
+  template <int U>
+  __device__ __forceinline__ static void load_group(CompressedStreamTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+    const uint8_t* lsb = reinterpret_cast<const uint8_t*>(c.line_status) + (t0 >> 1);
+    const uint32_t sh = 4u * (uint32_t)(t0 & 1);
+#pragma unroll
+    for (int m = 0; m < U; m++) {
+      const uint64_t t = t0 + (uint64_t)m * stride;
+      if (!PUSHDOWN)
+        g[m].sd = nt_load(reinterpret_cast<const uint2*>(c.ship_date) + t);
+      else
+        g[m].flt = (reinterpret_cast<const uint8_t*>(c.filter)[(t0 >> 1) + (uint64_t)m * (stride >> 1)] >> sh) & 0xFu;
+      g[m].q = nt_load(reinterpret_cast<const uint32_t*>(c.quantity) + t);
+      g[m].d = nt_load(reinterpret_cast<const uint32_t*>(c.discount) + t);
+      g[m].x = nt_load(reinterpret_cast<const uint32_t*>(c.tax) + t);
+      g[m].p = nt_load(reinterpret_cast<const uint4*>(c.extended_price) + t);
+      g[m].rf = reinterpret_cast<const uint8_t*>(c.return_flag)[t];
+      g[m].ls = (lsb[(uint64_t)m * (stride >> 1)] >> sh) & 0xFu;
+    }
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -366,6 +427,15 @@ struct UncompressedTile {
     }
     return qmax <= 6300u && phi == 0 && (plo & 0xFF000000u) == 0 && ((d.x | d.y | x.x | x.y) & 0xF0F0F0F0u) == 0;
   }
+  // Character flags encode to slots 0..5 only.
+  __device__ __forceinline__ static constexpr uint32_t bad_codes() { return 0; }
+  __device__ __forceinline__ void drop_bad_codes() {}
+  template <int U>
+  __device__ __forceinline__ static void load_group(UncompressedTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+#pragma unroll
+    for (int m = 0; m < U; m++) g[m].load(c, t0 + (uint64_t)m * stride);
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -442,6 +512,15 @@ struct UncompressedStreamTile {
   __device__ __forceinline__ bool in_domain() const {
     return max(q.x, q.y) <= 6300u && (p.y | p.w) == 0 && ((p.x | p.z) & 0xFF000000u) == 0 &&
            ((d | x) & 0xF0F0u) == 0;
+  }
+  // Character flags encode to slots 0..5 only.
+  __device__ __forceinline__ static constexpr uint32_t bad_codes() { return 0; }
+  __device__ __forceinline__ void drop_bad_codes() {}
+  template <int U>
+  __device__ __forceinline__ static void load_group(UncompressedStreamTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+#pragma unroll
+    for (int m = 0; m < U; m++) g[m].load(c, t0 + (uint64_t)m * stride);
   }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
@@ -533,6 +612,15 @@ struct ReferenceTile {
     }
     return hi == 0 && qmax <= 6300u && (plo & 0xFF000000u) == 0 && (dx & ~15u) == 0;
   }
+  // Character flags encode to slots 0..5 only.
+  __device__ __forceinline__ static constexpr uint32_t bad_codes() { return 0; }
+  __device__ __forceinline__ void drop_bad_codes() {}
+  template <int U>
+  __device__ __forceinline__ static void load_group(ReferenceTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+#pragma unroll
+    for (int m = 0; m < U; m++) g[m].load(c, t0 + (uint64_t)m * stride);
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -613,6 +701,15 @@ struct ReferenceStreamTile {
     return hi == 0 && max(q.x, q.z) <= 6300u && ((p.x | p.z) & 0xFF000000u) == 0 &&
            ((d.x | d.z | x.x | x.z) & ~15u) == 0;
   }
+  // Character flags encode to slots 0..5 only.
+  __device__ __forceinline__ static constexpr uint32_t bad_codes() { return 0; }
+  __device__ __forceinline__ void drop_bad_codes() {}
+  template <int U>
+  __device__ __forceinline__ static void load_group(ReferenceStreamTile (&g)[U], const q1_columns& c, uint64_t t0,
+                                                    uint64_t stride) {
+#pragma unroll
+    for (int m = 0; m < U; m++) g[m].load(c, t0 + (uint64_t)m * stride);
+  }
   template <bool EDGE>
   __device__ __forceinline__ FastRow fast(int j, int32_t cut, uint32_t valid) const {
     FastRow r;
@@ -657,6 +754,11 @@ struct TileGroup {
   uint32_t valid[U];  // rows of the tile inside [row_begin, row_end)
   bool use[U], edge[U];
   bool plain;         // every tile used and whole: loaded without any branch
+  // Straight-line loads of all U tiles, for a group the caller knows to be whole and inside
+  // the range for every lane.
+  __device__ __forceinline__ void load_plain(const q1_columns& c, uint64_t t0, uint64_t stride) {
+    Tile::template load_group<U>(tile, c, t0, stride);
+  }
   __device__ __forceinline__ void load(const q1_columns& c, uint64_t t0, uint64_t stride, uint32_t count,
                                        uint64_t end_tile, uint64_t row_begin, uint64_t row_end) {
     constexpr int TILE = Tile::TILE;
@@ -882,7 +984,8 @@ struct AccShared {
     for (int i = 0; i < 4 * kNumGroups; i++) tab[i * B + threadIdx.x] = 0;
   }
   __device__ __forceinline__ void add(const FastRow& r) {
-    if (r.counts()) {
+    // Nearly every row passes: lay the atomics out inline, with no branch around them.
+    if (__builtin_expect(r.counts(), 1)) {
       uint64_t* base = tab + r.gid * (4 * B) + threadIdx.x;
       __hip_atomic_fetch_add(base, pack_small64(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (NARROW)
@@ -1090,6 +1193,56 @@ __device__ void block_merge(const LaneTotals& t, uint64_t* red, q1_device_aggreg
 }
 
 // ------------------------------------------------------------------------------------
+// Row bodies of one tile group.
+// ------------------------------------------------------------------------------------
+// A whole group inside the range.  Rows with the unused return-flag code are first made
+// to fail the filter (a wavefront-uniform branch that real data never takes), so every
+// row's slot is below 6 and the slot test drops out of the row body.
+template <class Tile, int U, class Acc>
+__device__ __forceinline__ void add_whole_group(TileGroup<Tile, U>& grp, Acc& acc, int32_t cut) {
+  constexpr int TILE = Tile::TILE;
+  uint32_t bad = 0;
+#pragma unroll
+  for (int m = 0; m < U; m++) bad |= grp.tile[m].bad_codes();
+  if (ballot(bad != 0)) {
+#pragma unroll
+    for (int m = 0; m < U; m++) grp.tile[m].drop_bad_codes();
+  }
+#pragma unroll
+  for (int m = 0; m < U; m++)
+#pragma unroll
+    for (int j = 0; j < TILE; j++) {
+      const FastRow r = grp.tile[m].template fast<false>(j, cut, 0);
+      __builtin_assume(r.gid < kNumGroups);
+      acc.add(r);
+    }
+}
+
+// Any group: tiles may be unused, or partial at the range's ends.
+template <class Tile, int U, class Acc>
+__device__ __forceinline__ void add_any_group(const TileGroup<Tile, U>& grp, Acc& acc, int32_t cut) {
+  constexpr int TILE = Tile::TILE;
+  if (grp.plain) {
+#pragma unroll
+    for (int m = 0; m < U; m++)
+#pragma unroll
+      for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+  } else {
+#pragma unroll
+    for (int m = 0; m < U; m++) {
+      if (!grp.use[m]) continue;
+      if (!grp.edge[m]) {
+#pragma unroll
+        for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+      } else {
+#pragma unroll
+        for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------
 // The fast scan kernel: layout L, placement P.
 // ------------------------------------------------------------------------------------
 template <int L, int P, int B>
@@ -1116,29 +1269,75 @@ scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
   acc.init(reinterpret_cast<char*>(lds));
 
-  for (uint32_t k = 0, n = 0; k < tiles_per_thread; k += U, n++) {
-    TileGroup<Tile, U> grp;
-    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
-             a.row_end);
-    if (grp.plain) {
-#pragma unroll
-      for (int m = 0; m < U; m++)
-#pragma unroll
-        for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
-    } else {
-#pragma unroll
-      for (int m = 0; m < U; m++) {
-        if (!grp.use[m]) continue;
-        if (!grp.edge[m]) {
-#pragma unroll
-          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
-        } else {
-#pragma unroll
-          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
-        }
+  constexpr bool kSplit = P == Q1_PLACE_PER_THREAD_SHARED_MEM || P == kPlaceSharedNarrow;
+  if constexpr (kSplit) {
+    // Groups k in [k_lo, k_end) are whole and inside the range for every lane of the
+    // block: the tiles of group k are base + (k + m) * B + lane, and they all lie in
+    // [whole_begin, whole_end), the tiles with no row outside [row_begin, row_end).  They
+    // run a branch-free body; the few others (the range's first and last groups) a
+    // guarded one.  A dropped row's ship date 0xFFFF must fail the filter: compressed
+    // cutoffs of 65535 and above (beyond the year 2170) take the guarded body throughout.
+    const uint64_t whole_begin = (a.row_begin + TILE - 1) / TILE, whole_end = a.row_end / TILE;
+    uint32_t k_lo = 0, k_end = 0;
+    if ((L != Q1_LAYOUT_COMPRESSED || cut < 65535) && whole_end > base) {
+      const uint64_t lo = base >= whole_begin ? 0 : (whole_begin - base + B - 1) / B;
+      const uint64_t klo = (lo + U - 1) / U * U;
+      uint64_t kmax = (whole_end - base) / B;  // groups with k + U <= kmax are inside
+      if (kmax > tiles_per_thread) kmax = tiles_per_thread;
+      if (kmax >= klo + U) {
+        k_lo = (uint32_t)klo;
+        k_end = (uint32_t)(klo + (kmax - klo) / U * U);
       }
     }
-    if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
+    uint32_t k = 0, n = 0;
+    for (; k < k_lo; k += U, n++) {
+      TileGroup<Tile, U> grp;
+      grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+               a.row_end);
+      add_any_group(grp, acc, cut);
+      if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
+    }
+    for (; k < k_end; k += U, n++) {
+      TileGroup<Tile, U> grp;
+      grp.load_plain(a.cols, base + (uint64_t)k * B + threadIdx.x, B);
+      add_whole_group(grp, acc, cut);
+      if ((n + 1) % FLUSH == 0) acc.flush();
+    }
+    for (; k < tiles_per_thread; k += U, n++) {
+      TileGroup<Tile, U> grp;
+      grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+               a.row_end);
+      add_any_group(grp, acc, cut);
+      if ((n + 1) % FLUSH == 0) acc.flush();
+    }
+  } else {
+    // The register-table placements keep one guarded loop: a second copy of the row body
+    // costs them registers (spills, lower occupancy) that the saved instructions do not
+    // repay.
+    for (uint32_t k = 0, n = 0; k < tiles_per_thread; k += U, n++) {
+      TileGroup<Tile, U> grp;
+      grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+               a.row_end);
+      if (grp.plain) {
+#pragma unroll
+        for (int m = 0; m < U; m++)
+#pragma unroll
+          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+      } else {
+#pragma unroll
+        for (int m = 0; m < U; m++) {
+          if (!grp.use[m]) continue;
+          if (!grp.edge[m]) {
+#pragma unroll
+            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
+          } else {
+#pragma unroll
+            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
+          }
+        }
+      }
+      if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
+    }
   }
   acc.flush();
 
