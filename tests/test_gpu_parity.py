@@ -348,3 +348,35 @@ def test_unused_return_flag_code_never_counts(ctx, oracle, pattern):
     assert as_dicts(ctx.run(t, 5, n - 9, cutoff, q1.PER_THREAD_SHARED_MEM)) == \
         [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 5, n - 4, cutoff)]
     t.free()
+
+
+@pytest.mark.parametrize("tpt", [16, 112, 128])
+@pytest.mark.parametrize("mix", [False, True])
+def test_shared_packed_slot_bounds(ctx, tpt, mix):
+    """At most 128 rows per lane the LDS placement keeps a lane's price, count, discount and
+    quantity sums in one 64-bit slot (price bits [0,31), count [31,39), discount [39,50),
+    quantity [50,63)).  128 rows of the domain's largest values in one group fill each field
+    to just below its top: price 128 * (2^24 - 1) < 2^31, discount 128 * 15 < 2^11, quantity
+    128 * 63 < 2^13.  Closed form, with partial blocks."""
+    price = (1 << 24) - 1
+    n = 256 * tpt * 3 + 77
+    rows = np.arange(n)
+    # every row in one group (each lane's slot of that group takes all of its rows), or mixed
+    rf = (rows // 7) % 3 if mix else np.full(n, 2)
+    ls = (rows // 5) % 2 if mix else np.zeros(n, np.int64)
+    cols = {
+        "ship_date": np.zeros(n, np.uint16), "quantity": np.full(n, 63, np.uint8),
+        "extended_price": np.full(n, price, np.uint32), "discount": np.full(n, 15, np.uint8),
+        "tax": np.full(n, 15, np.uint8),
+        "return_flag": np.zeros((n + 15) // 16, np.uint32), "line_status": np.zeros((n + 31) // 32, np.uint32),
+    }
+    for i in range(n):
+        cols["return_flag"][i // 16] |= np.uint32(int(rf[i]) << (2 * (i % 16)))
+        cols["line_status"][i // 32] |= np.uint32(int(ls[i]) << (i % 32))
+    t = ctx.upload(q1.COMPRESSED, cols, n)
+    got = ctx.run(t, 0, n, 729999, q1.PER_THREAD_SHARED_MEM, q1.make_launch(256, tpt))
+    for g in range(6):
+        k = int(np.sum((rf << 1 | ls) == g))
+        assert (got[g].count, got[g].sum_qty, got[g].sum_base_price, got[g].sum_disc) == (k, k * 6300, k * price, k * 15), g
+        assert got[g].sum_disc_price == k * 85 * price and got[g].sum_charge == k * 85 * 115 * price, g
+    t.free()

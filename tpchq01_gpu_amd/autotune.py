@@ -25,7 +25,7 @@ import time
 from . import PACKAGE_DIR, q1
 
 THREADS = [64, 128, 256]
-TUPLES = [256, 512, 1024, 2048, 4096, 8192, 16384]
+TUPLES = [128, 256, 512, 1024, 2048, 4096, 8192, 16384]
 MALL_BYTES = 256 << 20
 
 

@@ -1020,6 +1020,50 @@ struct AccShared {
   static constexpr uint32_t lds_bytes(uint32_t) { return shared_table_bytes(B); }
 };
 
+// Variant of the LDS table for compressed-layout launches of at most 128 rows per lane
+// (quantities in whole units, at most 63): three slots per group instead of four.  A lane's price sum (< 128 * 2^24 = 2^31) shares one 64-bit slot
+// with its count (<= 128), discount sum (<= 128 * 15 < 2^11) and quantity sum
+// (<= 128 * 63 < 2^13): price bits [0,31), count [31,39), discount [39,50), quantity
+// [50,63).  Three return-less LDS atomics per passing row instead of four, and a table of
+// 6 * 3 * 8 bytes per lane, so four 256-thread blocks fit a compute unit's LDS.
+constexpr uint32_t packed_table_bytes(uint32_t threads) { return 3u * kNumGroups * 8u * threads; }
+
+template <int B>
+struct AccSharedPacked {
+  uint64_t* tab;  // dynamic LDS: 6 * 3 * B words
+  __device__ __forceinline__ void init(char* lds) {
+    tab = reinterpret_cast<uint64_t*>(lds);
+#pragma unroll
+    for (int i = 0; i < 3 * kNumGroups; i++) tab[i * B + threadIdx.x] = 0;
+  }
+  __device__ __forceinline__ void add(const FastRow& r) {
+    // Nearly every row passes: lay the atomics out inline, with no branch around them.
+    if (__builtin_expect(r.counts(), 1)) {
+      uint64_t* base = tab + r.gid * (3 * B) + threadIdx.x;
+      const uint64_t w = (uint64_t)(r.price | 0x80000000u) | ((uint64_t)(r.disc << 7 | r.qty << 18) << 32);
+      __hip_atomic_fetch_add(base, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + B, (uint64_t)disc_prod(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + 2 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __device__ __forceinline__ void flush() {}
+  static constexpr bool kPackedSmall = false;
+  __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      const uint64_t* base = tab + g * (3 * B) + threadIdx.x;
+      const uint64_t w = base[0];
+      o.v[g][F_PRICE] = w & 0x7FFFFFFFull;
+      o.v[g][F_CNT] = (w >> 31) & 0xFFull;
+      o.v[g][F_DISC] = (w >> 39) & 0x7FFull;
+      o.v[g][F_QTY] = w >> 50;
+      o.v[g][F_DPROD] = base[B];
+      o.v[g][F_CHARGE] = base[2 * B];
+    }
+  }
+  static constexpr uint32_t lds_bytes(uint32_t) { return packed_table_bytes(B); }
+};
+
 // Q1_PLACE_GLOBAL: one table per block in global memory (the kernel's workspace),
 // updated by every passing row with device-memory atomics.  The table is shared by all
 // lanes of the block, so it sees up to blockDim * 16384 rows: count and discount share one
@@ -1075,6 +1119,9 @@ template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
 // Internal kernel variant of the shared placement for launches of at most 256 rows per lane.
 constexpr int kPlaceSharedNarrow = 8;
 template <int B> struct AccOf<kPlaceSharedNarrow, B> { using A = AccShared<B, true>; };
+// ... and of at most 128 rows per lane, with three slots per group.
+constexpr int kPlaceSharedPacked = 9;
+template <int B> struct AccOf<kPlaceSharedPacked, B> { using A = AccSharedPacked<B>; };
 
 template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED, Q1_PLACE_PER_THREAD_SHARED_MEM> { using T = CompressedStreamTile<false>; };
 template <> struct ScanTileOf<Q1_LAYOUT_COMPRESSED, kPlaceSharedNarrow> { using T = CompressedStreamTile<false>; };
@@ -1086,6 +1133,7 @@ template <> struct ScanTileOf<Q1_LAYOUT_UNCOMPRESSED, Q1_PLACE_PER_THREAD_SHARED
 template <> struct ScanTileOf<Q1_LAYOUT_UNCOMPRESSED, kPlaceSharedNarrow> { using T = UncompressedStreamTile; };
 template <> struct ScanTileOf<Q1_LAYOUT_REFERENCE, Q1_PLACE_PER_THREAD_SHARED_MEM> { using T = ReferenceStreamTile; };
 template <> struct ScanTileOf<Q1_LAYOUT_REFERENCE, kPlaceSharedNarrow> { using T = ReferenceStreamTile; };
+template <int L> struct ScanTileOf<L, kPlaceSharedPacked> { using T = typename ScanTileOf<L, kPlaceSharedNarrow>::T; };
 
 // ------------------------------------------------------------------------------------
 // Block merge and the single global merge per block.
@@ -1269,7 +1317,7 @@ scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
   acc.init(reinterpret_cast<char*>(lds));
 
-  constexpr bool kSplit = P == Q1_PLACE_PER_THREAD_SHARED_MEM || P == kPlaceSharedNarrow;
+  constexpr bool kSplit = P == Q1_PLACE_PER_THREAD_SHARED_MEM || P == kPlaceSharedNarrow || P == kPlaceSharedPacked;
   if constexpr (kSplit) {
     // Groups k in [k_lo, k_end) are whole and inside the range for every lane of the
     // block: the tiles of group k are base + (k + m) * B + lane, and they all lie in
@@ -1771,6 +1819,7 @@ constexpr FastFn fast_fn_b(int p) {
        : p == Q1_PLACE_LOCAL_MEM ? scan_fast<L, Q1_PLACE_LOCAL_MEM, B>
        : p == Q1_PLACE_PER_THREAD_SHARED_MEM ? scan_fast<L, Q1_PLACE_PER_THREAD_SHARED_MEM, B>
        : p == kPlaceSharedNarrow ? scan_fast<L, kPlaceSharedNarrow, B>
+       : p == kPlaceSharedPacked ? scan_fast<L, kPlaceSharedPacked, B>
        : scan_fast<L, Q1_PLACE_GLOBAL, B>;
 }
 
@@ -1877,11 +1926,17 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
       a.out = spread->tables;
       a.out_mask = kSpreadTables - 1;
     }
-    uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? shared_table_bytes(shape.threads) : 0;
+    const uint64_t lane_rows = (uint64_t)shape.tiles_per_thread * tile_rows(layout);
+    const bool shared = placement == Q1_PLACE_PER_THREAD_SHARED_MEM;
+    // The packed slot holds quantities in whole units: the compressed layouts only.
+    const bool units = layout == Q1_LAYOUT_COMPRESSED || layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN;
+    const int variant = shared && units && lane_rows <= 128 ? kPlaceSharedPacked
+                        : shared && lane_rows <= 256        ? kPlaceSharedNarrow
+                                                            : (int)placement;
+    uint32_t lds = variant == kPlaceSharedPacked ? packed_table_bytes(shape.threads)
+                   : shared                     ? shared_table_bytes(shape.threads)
+                                                : 0;
     lds += nwaves * kNumGroups * NF * 8;
-    const bool narrow = placement == Q1_PLACE_PER_THREAD_SHARED_MEM &&
-                        (uint64_t)shape.tiles_per_thread * tile_rows(layout) <= 256;
-    const int variant = narrow ? kPlaceSharedNarrow : (int)placement;
     hipLaunchKernelGGL(fast_kernel(layout, variant, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds,
                        s, a, shape.tiles_per_thread, (uint64_t*)workspace);
     if (spread_merge) {
