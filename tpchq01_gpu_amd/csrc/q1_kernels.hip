@@ -1047,16 +1047,18 @@ struct AccSharedPacked {
     }
   }
   __device__ __forceinline__ void flush() {}
-  static constexpr bool kPackedSmall = false;
+  // The block merge gets count, discount and quantity re-packed in the pack_small64 layout
+  // (a wavefront's sums of them fit it: count <= 64 * 128, discount <= 64 * 1920,
+  // quantity <= 64 * 8064), so it sums one word for the three.
+  static constexpr bool kPackedSmall = true;
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       const uint64_t* base = tab + g * (3 * B) + threadIdx.x;
       const uint64_t w = base[0];
       o.v[g][F_PRICE] = w & 0x7FFFFFFFull;
-      o.v[g][F_CNT] = (w >> 31) & 0xFFull;
-      o.v[g][F_DISC] = (w >> 39) & 0x7FFull;
-      o.v[g][F_QTY] = w >> 50;
+      o.v[g][F_CNT] = ((w >> 31) & 0xFFull) | ((w >> 39) & 0x7FFull) << 17 | (w >> 50) << 37;
+      o.v[g][F_QTY] = o.v[g][F_DISC] = 0;
       o.v[g][F_DPROD] = base[B];
       o.v[g][F_CHARGE] = base[2 * B];
     }
