@@ -350,14 +350,15 @@ def test_unused_return_flag_code_never_counts(ctx, oracle, pattern):
     t.free()
 
 
-@pytest.mark.parametrize("tpt", [16, 112, 128])
+@pytest.mark.parametrize("tpt", [16, 128, 240, 256])
 @pytest.mark.parametrize("mix", [False, True])
 def test_shared_packed_slot_bounds(ctx, tpt, mix):
-    """At most 128 rows per lane the LDS placement keeps a lane's price, count, discount and
-    quantity sums in one 64-bit slot (price bits [0,31), count [31,39), discount [39,50),
-    quantity [50,63)).  128 rows of the domain's largest values in one group fill each field
-    to just below its top: price 128 * (2^24 - 1) < 2^31, discount 128 * 15 < 2^11, quantity
-    128 * 63 < 2^13.  Closed form, with partial blocks."""
+    """At most 256 rows per lane the LDS placement keeps a compressed lane's sums in three
+    64-bit slots: price [0,32), discount [32,44) and quantity [44,58) in one; the discount
+    product [0,36) and count [48,57) in another; the charge.  256 rows of the domain's
+    largest values in one group fill each field to just below its top: price
+    256 * (2^24 - 1) < 2^32, discount 256 * 15 < 2^12, quantity 256 * 63 < 2^14, product
+    256 * 15 * (2^24 - 1) < 2^36.  Closed form, with partial blocks."""
     price = (1 << 24) - 1
     n = 256 * tpt * 3 + 77
     rows = np.arange(n)

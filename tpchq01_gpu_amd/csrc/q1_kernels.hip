@@ -1020,11 +1020,14 @@ struct AccShared {
   static constexpr uint32_t lds_bytes(uint32_t) { return shared_table_bytes(B); }
 };
 
-// Variant of the LDS table for compressed-layout launches of at most 128 rows per lane
-// (quantities in whole units, at most 63): three slots per group instead of four.  A lane's price sum (< 128 * 2^24 = 2^31) shares one 64-bit slot
-// with its count (<= 128), discount sum (<= 128 * 15 < 2^11) and quantity sum
-// (<= 128 * 63 < 2^13): price bits [0,31), count [31,39), discount [39,50), quantity
-// [50,63).  Three return-less LDS atomics per passing row instead of four, and a table of
+// Variant of the LDS table for compressed-layout launches of at most 256 rows per lane
+// (quantities in whole units, at most 63): three slots per group instead of four.
+//   slot 0: price sum (< 256 * 2^24 = 2^32) in bits [0,32), discount sum (<= 256 * 15
+//           < 2^12) in [32,44), quantity sum (<= 256 * 63 < 2^14) in [44,58);
+//   slot 1: discount-price product sum (< 256 * 15 * 2^24 < 2^36) in [0,36), count
+//           (<= 256) in [48,57);
+//   slot 2: charge.
+// Three return-less 64-bit LDS atomics per passing row instead of four, and a table of
 // 6 * 3 * 8 bytes per lane, so four 256-thread blocks fit a compute unit's LDS.
 constexpr uint32_t packed_table_bytes(uint32_t threads) { return 3u * kNumGroups * 8u * threads; }
 
@@ -1040,26 +1043,27 @@ struct AccSharedPacked {
     // Nearly every row passes: lay the atomics out inline, with no branch around them.
     if (__builtin_expect(r.counts(), 1)) {
       uint64_t* base = tab + r.gid * (3 * B) + threadIdx.x;
-      const uint64_t w = (uint64_t)(r.price | 0x80000000u) | ((uint64_t)(r.disc << 7 | r.qty << 18) << 32);
-      __hip_atomic_fetch_add(base, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      __hip_atomic_fetch_add(base + B, (uint64_t)disc_prod(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint64_t w0 = (uint64_t)r.price | (uint64_t)(r.disc | r.qty << 12) << 32;
+      const uint64_t w1 = (uint64_t)disc_prod(r) | 1ull << 48;
+      __hip_atomic_fetch_add(base, w0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(base + B, w1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       __hip_atomic_fetch_add(base + 2 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
   __device__ __forceinline__ void flush() {}
   // The block merge gets count, discount and quantity re-packed in the pack_small64 layout
-  // (a wavefront's sums of them fit it: count <= 64 * 128, discount <= 64 * 1920,
-  // quantity <= 64 * 8064), so it sums one word for the three.
+  // (a wavefront's sums of them fit it: count <= 64 * 256, discount <= 64 * 3840,
+  // quantity <= 64 * 16128), so it sums one word for the three.
   static constexpr bool kPackedSmall = true;
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       const uint64_t* base = tab + g * (3 * B) + threadIdx.x;
-      const uint64_t w = base[0];
-      o.v[g][F_PRICE] = w & 0x7FFFFFFFull;
-      o.v[g][F_CNT] = ((w >> 31) & 0xFFull) | ((w >> 39) & 0x7FFull) << 17 | (w >> 50) << 37;
+      const uint64_t w0 = base[0], w1 = base[B];
+      o.v[g][F_PRICE] = w0 & 0xFFFFFFFFull;
+      o.v[g][F_CNT] = (w1 >> 48) | ((w0 >> 32) & 0xFFFull) << 17 | (w0 >> 44) << 37;
       o.v[g][F_QTY] = o.v[g][F_DISC] = 0;
-      o.v[g][F_DPROD] = base[B];
+      o.v[g][F_DPROD] = w1 & ((1ull << 48) - 1);
       o.v[g][F_CHARGE] = base[2 * B];
     }
   }
@@ -1121,7 +1125,7 @@ template <int B> struct AccOf<Q1_PLACE_GLOBAL, B> { using A = AccGlobal; };
 // Internal kernel variant of the shared placement for launches of at most 256 rows per lane.
 constexpr int kPlaceSharedNarrow = 8;
 template <int B> struct AccOf<kPlaceSharedNarrow, B> { using A = AccShared<B, true>; };
-// ... and of at most 128 rows per lane, with three slots per group.
+// ... and, for the compressed layouts, the packed table with three slots per group.
 constexpr int kPlaceSharedPacked = 9;
 template <int B> struct AccOf<kPlaceSharedPacked, B> { using A = AccSharedPacked<B>; };
 
@@ -1932,7 +1936,7 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
     const bool shared = placement == Q1_PLACE_PER_THREAD_SHARED_MEM;
     // The packed slot holds quantities in whole units: the compressed layouts only.
     const bool units = layout == Q1_LAYOUT_COMPRESSED || layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN;
-    const int variant = shared && units && lane_rows <= 128 ? kPlaceSharedPacked
+    const int variant = shared && units && lane_rows <= 256 ? kPlaceSharedPacked
                         : shared && lane_rows <= 256        ? kPlaceSharedNarrow
                                                             : (int)placement;
     uint32_t lds = variant == kPlaceSharedPacked ? packed_table_bytes(shape.threads)
