@@ -1430,13 +1430,75 @@ struct Wide128 {
   }
 };
 
+// The sums of one slot for rows outside the TPC-H domain: 64-bit (wrapping, as the
+// reference's int64 sums) and 128-bit products.
+struct WideSums {
+  uint64_t qty, price, disc, cnt;
+  Wide128 dp, ch;
+};
+
+template <class Tile>
+__device__ __forceinline__ void add_wide_rows(const Tile& tile, WideSums* wide, int32_t cut, uint32_t valid) {
+#pragma unroll
+  for (int j = 0; j < Tile::TILE; j++) {
+    const WideRow r = tile.wide(j, cut, valid);
+    if (r.gid < kNumGroups) {
+      const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
+      const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
+      const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
+      WideSums& t = wide[r.gid];
+      t.qty += (uint64_t)r.qty;
+      t.price += (uint64_t)r.price;
+      t.disc += (uint64_t)r.disc;
+      t.cnt += 1;
+      t.dp.add64s(disc_price);
+      t.ch.add64s(charge);
+    }
+  }
+}
+
+// A group of tiles some of whose rows may lie outside the domain: each lane sends each of
+// its tiles to the narrow LDS table or to its wide table.
+template <class Tile, int U, class Acc>
+__device__ __forceinline__ void add_group_any_value(const TileGroup<Tile, U>& grp, Acc& narrow, WideSums* wide,
+                                                    bool& wide_used, int32_t cut, bool whole) {
+  constexpr int TILE = Tile::TILE;
+#pragma unroll
+  for (int m = 0; m < U; m++) {
+    if (!whole && !grp.use[m]) continue;
+    const Tile& tile = grp.tile[m];
+    const bool edge = !whole && grp.edge[m];
+    const uint32_t valid = whole ? (1u << TILE) - 1 : grp.valid[m];
+    if (tile.in_domain()) {
+      if (!edge) {
+#pragma unroll
+        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
+      } else {
+#pragma unroll
+        for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, valid));
+      }
+    } else {
+      if (!wide_used) {  // the lane's first row outside the domain
+#pragma unroll
+        for (int g = 0; g < kNumGroups; g++) wide[g] = WideSums{0, 0, 0, 0, {0, 0}, {0, 0}};
+        wide_used = true;
+      }
+      add_wide_rows(tile, wide, cut, valid);
+    }
+  }
+}
+
 template <int L, int B>
 __global__ void __launch_bounds__(B)
-scan_any(ScanArgs a, uint32_t tiles_per_thread) {
-  using Tile = typename TileOf<L>::T;
+scan_any(ScanArgs a, uint32_t host_tiles_per_thread) {
+  // The LDS placement's streaming tiles; the host counts tiles of TileOf<L>.
+  using Tile = typename ScanTileOf<L, kPlaceSharedNarrow>::T;
   constexpr int TILE = Tile::TILE;
+  constexpr int U = Tile::GROUP;
+  constexpr int SUB = TileOf<L>::T::TILE / TILE;
   extern __shared__ uint64_t lds[];
 
+  const uint32_t tiles_per_thread = host_tiles_per_thread * SUB;
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
   const uint64_t first_tile = a.row_begin / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
@@ -1444,6 +1506,50 @@ scan_any(ScanArgs a, uint32_t tiles_per_thread) {
 
   AccShared<B> narrow;
   narrow.init(reinterpret_cast<char*>(lds));
+  // Indexed by slot at run time, so it lives in the lane's private memory and costs no
+  // registers; only lanes with rows outside the domain clear, update and read it.
+  WideSums wide[kNumGroups];
+  bool wide_used = false;
+
+  // Whole groups as in scan_fast: when every row of a wavefront's group lies in the domain
+  // it takes the branch-free body; otherwise its lanes sort their tiles.
+  const uint64_t whole_begin = (a.row_begin + TILE - 1) / TILE, whole_end = a.row_end / TILE;
+  uint32_t k_lo = 0, k_end = 0;
+  if ((L != Q1_LAYOUT_COMPRESSED || cut < 65535) && whole_end > base) {
+    const uint64_t lo = base >= whole_begin ? 0 : (whole_begin - base + B - 1) / B;
+    const uint64_t klo = (lo + U - 1) / U * U;
+    uint64_t kmax = (whole_end - base) / B;
+    if (kmax > tiles_per_thread) kmax = tiles_per_thread;
+    if (kmax >= klo + U) {
+      k_lo = (uint32_t)klo;
+      k_end = (uint32_t)(klo + (kmax - klo) / U * U);
+    }
+  }
+  uint32_t k = 0;
+  for (; k < k_lo; k += U) {
+    TileGroup<Tile, U> grp;
+    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+             a.row_end);
+    add_group_any_value(grp, narrow, wide, wide_used, cut, false);
+  }
+  for (; k < k_end; k += U) {
+    TileGroup<Tile, U> grp;
+    grp.load_plain(a.cols, base + (uint64_t)k * B + threadIdx.x, B);
+    bool in = true;
+#pragma unroll
+    for (int m = 0; m < U; m++) in = in & grp.tile[m].in_domain();
+    if (ballot(!in) == 0)
+      add_whole_group(grp, narrow, cut);
+    else
+      add_group_any_value(grp, narrow, wide, wide_used, cut, true);
+  }
+  for (; k < tiles_per_thread; k += U) {
+    TileGroup<Tile, U> grp;
+    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+             a.row_end);
+    add_group_any_value(grp, narrow, wide, wide_used, cut, false);
+  }
+
   uint64_t qty[kNumGroups], price[kNumGroups], disc[kNumGroups], cnt[kNumGroups];
   Wide128 dp[kNumGroups], ch[kNumGroups];
 #pragma unroll
@@ -1451,44 +1557,15 @@ scan_any(ScanArgs a, uint32_t tiles_per_thread) {
     qty[g] = price[g] = disc[g] = cnt[g] = 0;
     dp[g].lo = dp[g].hi = ch[g].lo = ch[g].hi = 0;
   }
-
-  constexpr int U = Tile::GROUP;
-  for (uint32_t k = 0; k < tiles_per_thread; k += U) {
-    TileGroup<Tile, U> grp;
-    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
-             a.row_end);
+  if (wide_used) {
 #pragma unroll
-    for (int m = 0; m < U; m++) {
-      if (!grp.use[m]) continue;
-      const Tile& tile = grp.tile[m];
-      if (tile.in_domain()) {
-        if (!grp.edge[m]) {
-#pragma unroll
-          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
-        } else {
-#pragma unroll
-          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, grp.valid[m]));
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < TILE; j++) {
-          const WideRow r = tile.wide(j, cut, grp.valid[m]);
-          const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
-          const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
-          const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
-#pragma unroll
-          for (int g = 0; g < kNumGroups; g++) {
-            if (r.gid == (uint32_t)g) {
-              qty[g] += (uint64_t)r.qty;
-              price[g] += (uint64_t)r.price;
-              disc[g] += (uint64_t)r.disc;
-              cnt[g] += 1;
-              dp[g].add64s(disc_price);
-              ch[g].add64s(charge);
-            }
-          }
-        }
-      }
+    for (int g = 0; g < kNumGroups; g++) {
+      qty[g] = wide[g].qty;
+      price[g] = wide[g].price;
+      disc[g] = wide[g].disc;
+      cnt[g] = wide[g].cnt;
+      dp[g] = wide[g].dp;
+      ch[g] = wide[g].ch;
     }
   }
 
