@@ -258,3 +258,55 @@ def test_batch_small_grids_span_many_requests(ctx):
             for i in range(n):
                 assert q1.result_to_list(outs[i]) == want[i], (rows, blocks, i)
     t.free()
+
+
+@pytest.mark.parametrize("layout", [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED])
+@pytest.mark.parametrize("spread", ["scattered", "everywhere"])
+def test_enqueue_without_domain_flag_outside_the_domain(ctx, oracle, layout, spread):
+    """q1_enqueue without the domain flag (the reference's launch contract) runs the
+    any-value kernel that keeps out-of-domain sums in a per-lane private table, touched only
+    from a lane's first out-of-domain row: rows outside the domain in a few scattered rows
+    (most wavefront groups stay on the in-domain body) or in every fifth row (every lane
+    uses its private table).  Every placement and several shapes, exact vs the oracle."""
+    import torch
+    rows = 200_003
+    ref = q1.generate_host(q1.REFERENCE, SEED + 41, rows)
+    rng = np.random.default_rng(41)
+    bad = rng.choice(rows, 40, replace=False) if spread == "scattered" else np.arange(0, rows, 5)
+    k = len(bad) // 4
+    if layout == q1.COMPRESSED:
+        ref["discount"][bad[:k]] = 200
+        ref["tax"][bad[k:2 * k]] = 99
+        ref["extended_price"][bad[2 * k:3 * k]] = 3_000_000_000
+        ref["quantity"][bad[3 * k:]] = 9900
+        dev = q1.compress_columns(ref, rows)
+    elif layout == q1.UNCOMPRESSED:
+        ref["discount"][bad[:k]] = -5
+        ref["tax"][bad[k:2 * k]] = 100
+        ref["extended_price"][bad[2 * k:3 * k]] = 1 << 45
+        ref["quantity"][bad[3 * k:]] = 6301
+        dev = {"ship_date": ref["ship_date"], "quantity": ref["quantity"].astype(np.int32),
+               "extended_price": ref["extended_price"], "discount": ref["discount"].astype(np.int8),
+               "tax": ref["tax"].astype(np.int8), "return_flag": ref["return_flag"],
+               "line_status": ref["line_status"]}
+    else:
+        ref["discount"][bad[:k]] = -5
+        ref["tax"][bad[k:2 * k]] = 1 << 40
+        ref["extended_price"][bad[2 * k:3 * k]] = 1 << 45
+        ref["quantity"][bad[3 * k:]] = 6301
+        dev = ref
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(layout, dev, rows)
+    cols = t.device_columns()
+    agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+    for p in range(5):
+        for shape in (None, (64, 16), (256, 1024)):
+            note(f"case enqueue any-value layout={q1.LAYOUT_NAMES[layout]} {spread} placement={p} shape={shape}")
+            agg.zero_()
+            torch.cuda.synchronize()
+            l = q1.make_launch(*shape) if shape else None
+            ctx.check(q1.lib.q1_enqueue(ctx.handle, p, layout, C.byref(l) if l else None, C.c_void_p(agg.data_ptr()),
+                                        C.byref(cols), 0, rows, 729999, None), "q1_enqueue")
+            ctx.synchronize()
+            assert as_dicts(device_aggregates_to_result(agg)) == want, (p, shape)
+    t.free()

@@ -78,7 +78,7 @@ q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes) {
 }
 
 q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
-                  const LaunchShape& shape, void* stream) {
+                  const LaunchShape& shape, void* stream, bool table_outside) {
   // Launches on the context stream run one after another, so they can share one set of
   // spread tables; launches on other streams merge directly.
   const bool own = stream == (void*)ctx->stream && ctx->spread.tables;
@@ -88,7 +88,7 @@ q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fa
     ctx->spread_dirty = false;
   }
   const q1_status s = launch_scan(placement, layout, fast, a, shape, ctx->workspace, stream,
-                                  own ? &ctx->spread : nullptr);
+                                  own ? &ctx->spread : nullptr, table_outside);
   if (s != Q1_OK && own) ctx->spread_dirty = true;
   return s;
 }
@@ -596,7 +596,7 @@ static q1_status run_locked(q1_ctx* ctx, const q1_table* t, uint64_t row_begin, 
   s = launch_zero(ctx->d_aggr, 1, ctx->stream);
   if (s != Q1_OK) return fail(ctx, s, "zero kernel launch failed");
   Q1_HIP(ctx, hipEventRecord(ctx->ev_begin, ctx->stream));
-  s = scan_on(ctx, p, t->layout, t->fast, a, shape, ctx->stream);
+  s = scan_on(ctx, p, t->layout, t->fast, a, shape, ctx->stream, !t->fast);
   if (s != Q1_OK) return fail(ctx, s, "scan kernel launch failed");
   Q1_HIP(ctx, hipEventRecord(ctx->ev_end, ctx->stream));
   Q1_HIP(ctx, hipMemcpyAsync(ctx->h_aggr, ctx->d_aggr, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,

@@ -59,8 +59,9 @@ void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows
 uint64_t algorithmic_bytes(q1_layout layout, uint64_t rows);
 q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
 // launch_scan with the context's spread tables when `stream` is the context stream.
+// table_outside: the rows come from a table known to hold values outside the domain.
 q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
-                  const LaunchShape& shape, void* stream);
+                  const LaunchShape& shape, void* stream, bool table_outside = false);
 void batch_destroy(q1_batch_state* b);
 void stream_state_release(q1_ctx* ctx);
 }  // namespace q1

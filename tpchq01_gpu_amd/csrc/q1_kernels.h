@@ -48,8 +48,11 @@ q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* 
 // Enqueue the scan kernel.  `fast` selects the narrow (TPC-H domain) kernels.  With a
 // `spread` (may be null) a fast launch of at least kSpreadMinBlocks blocks merges through
 // the spread tables and is followed by the fold kernel on the same stream.
+// fast: the rows lie inside the TPC-H domain.  Otherwise an any-value kernel runs;
+// table_outside picks the one for tables known to hold values outside the domain.
 q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
-                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread = nullptr);
+                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread = nullptr,
+                      bool table_outside = false);
 uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape);
 
 // Zero a device aggregate table (a fill kernel, safe inside graph capture).

@@ -1643,6 +1643,146 @@ scan_any(ScanArgs a, uint32_t host_tiles_per_thread) {
   }
 }
 
+// The any-value kernel for tables known to hold values outside the domain (q1_run and the
+// server on such tables): 16-row tiles, each lane's tile tested as above, and the wide
+// sums in registers, so a table with every row outside the domain still runs at a few
+// times the fast kernel's time instead of the private-memory table's.
+template <int L, int B>
+__global__ void __launch_bounds__(B)
+scan_wide(ScanArgs a, uint32_t tiles_per_thread) {
+  using Tile = typename TileOf<L>::T;
+  constexpr int TILE = Tile::TILE;
+  extern __shared__ uint64_t lds[];
+
+  const int32_t cut = column_cutoff<L>(a.cutoff_day);
+  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
+  const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
+
+  AccShared<B> narrow;
+  narrow.init(reinterpret_cast<char*>(lds));
+  uint64_t qty[kNumGroups], price[kNumGroups], disc[kNumGroups], cnt[kNumGroups];
+  Wide128 dp[kNumGroups], ch[kNumGroups];
+#pragma unroll
+  for (int g = 0; g < kNumGroups; g++) {
+    qty[g] = price[g] = disc[g] = cnt[g] = 0;
+    dp[g].lo = dp[g].hi = ch[g].lo = ch[g].hi = 0;
+  }
+
+  constexpr int U = Tile::GROUP;
+  for (uint32_t k = 0; k < tiles_per_thread; k += U) {
+    TileGroup<Tile, U> grp;
+    grp.load(a.cols, base + (uint64_t)k * B + threadIdx.x, B, tiles_per_thread - k, end_tile, a.row_begin,
+             a.row_end);
+#pragma unroll
+    for (int m = 0; m < U; m++) {
+      if (!grp.use[m]) continue;
+      const Tile& tile = grp.tile[m];
+      if (tile.in_domain()) {
+        if (!grp.edge[m]) {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<false>(j, cut, 0));
+        } else {
+#pragma unroll
+          for (int j = 0; j < TILE; j++) narrow.add(tile.template fast<true>(j, cut, grp.valid[m]));
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < TILE; j++) {
+          const WideRow r = tile.wide(j, cut, grp.valid[m]);
+          const int64_t disc_1 = (int64_t)(100ull - (uint64_t)r.disc);
+          const int64_t disc_price = (int64_t)((uint64_t)disc_1 * (uint64_t)r.price);
+          const int64_t charge = (int64_t)((uint64_t)disc_price * (uint64_t)(r.tax + 100));
+#pragma unroll
+          for (int g = 0; g < kNumGroups; g++) {
+            if (r.gid == (uint32_t)g) {
+              qty[g] += (uint64_t)r.qty;
+              price[g] += (uint64_t)r.price;
+              disc[g] += (uint64_t)r.disc;
+              cnt[g] += 1;
+              dp[g].add64s(disc_price);
+              ch[g].add64s(charge);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // Fold the lane's narrow table into its wide one.  Lane sums of the narrow table are
+  // exact and far below 2^63 (at most 16384 rows per lane).
+  {
+    LaneTotals nt;
+    narrow.totals(nt, reinterpret_cast<char*>(lds));
+#pragma unroll
+    for (int g = 0; g < kNumGroups; g++) {
+      qty[g] += nt.v[g][F_QTY];
+      price[g] += nt.v[g][F_PRICE];
+      disc[g] += nt.v[g][F_DISC];
+      cnt[g] += nt.v[g][F_CNT];
+      dp[g].add64s((int64_t)(100u * nt.v[g][F_PRICE] - nt.v[g][F_DPROD]));
+      ch[g].add64s((int64_t)nt.v[g][F_CHARGE]);
+    }
+  }
+
+  // Block merge in 128 bits for every field; the scratch follows the narrow table.
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = B >> 6;
+  constexpr int NW = 8;  // qty, price, disc, cnt, dp.lo, dp.hi, ch.lo, ch.hi
+  uint64_t* red = lds + shared_table_bytes(B) / 8;  // nwaves * 6 * NW words
+#pragma unroll
+  for (int g = 0; g < kNumGroups; g++) {
+    uint64_t s64[4] = {qty[g], price[g], disc[g], cnt[g]};
+    Wide128 w[2] = {dp[g], ch[g]};
+    if (ballot(cnt[g] != 0)) {
+#pragma unroll
+      for (int f = 0; f < 4; f++) s64[f] = wave_sum_u64(s64[f]);
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+          const uint64_t olo = __shfl_xor(w[f].lo, m, 64), ohi = __shfl_xor(w[f].hi, m, 64);
+          const uint64_t nlo = w[f].lo + olo;
+          w[f].hi += ohi + (nlo < olo);
+          w[f].lo = nlo;
+        }
+      }
+    } else {
+      s64[0] = s64[1] = s64[2] = s64[3] = 0;
+      w[0].lo = w[0].hi = w[1].lo = w[1].hi = 0;
+    }
+    if (lane == 0) {
+      uint64_t* r = red + (wave * kNumGroups + g) * NW;
+      r[0] = s64[0]; r[1] = s64[1]; r[2] = s64[2]; r[3] = s64[3];
+      r[4] = w[0].lo; r[5] = w[0].hi; r[6] = w[1].lo; r[7] = w[1].hi;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumGroups) {
+    const int g = threadIdx.x;
+    uint64_t s[4] = {0, 0, 0, 0};
+    Wide128 w[2] = {{0, 0}, {0, 0}};
+    for (uint32_t v = 0; v < nwaves; v++) {
+      const uint64_t* r = red + (v * kNumGroups + g) * NW;
+      for (int f = 0; f < 4; f++) s[f] += r[f];
+      for (int f = 0; f < 2; f++) {
+        const uint64_t nlo = w[f].lo + r[4 + 2 * f];
+        w[f].hi += r[5 + 2 * f] + (nlo < w[f].lo);
+        w[f].lo = nlo;
+      }
+    }
+    if (s[3]) {
+      const uint64_t qs =
+          (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
+      atomicAdd((unsigned long long*)&a.out->sum_qty[g], (unsigned long long)(s[0] * qs));
+      atomicAdd((unsigned long long*)&a.out->sum_base_price[g], (unsigned long long)s[1]);
+      atomicAdd((unsigned long long*)&a.out->sum_disc[g], (unsigned long long)s[2]);
+      atomicAdd((unsigned long long*)&a.out->count[g], (unsigned long long)s[3]);
+      atomic_add128(&a.out->sum_disc_price_lo[g], &a.out->sum_disc_price_hi[g], w[0].lo, w[0].hi);
+      atomic_add128(&a.out->sum_charge_lo[g], &a.out->sum_charge_hi[g], w[1].lo, w[1].hi);
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // Utility kernels: zero fill, generator, domain statistics, pushdown producer.
 // ------------------------------------------------------------------------------------
@@ -1925,12 +2065,21 @@ GenFn any_fn(uint32_t threads) {
   return threads == 64 ? scan_any<L, 64> : threads == 128 ? scan_any<L, 128> : scan_any<L, 256>;
 }
 
-GenFn general_kernel(q1_layout l, uint32_t threads) {
+template <int L>
+GenFn wide_fn(uint32_t threads) {
+  return threads == 64 ? scan_wide<L, 64> : threads == 128 ? scan_wide<L, 128> : scan_wide<L, 256>;
+}
+
+// outside: the table is known to hold values outside the domain.
+GenFn general_kernel(q1_layout l, uint32_t threads, bool outside) {
   switch (l) {
-    case Q1_LAYOUT_REFERENCE: return any_fn<Q1_LAYOUT_REFERENCE>(threads);
-    case Q1_LAYOUT_UNCOMPRESSED: return any_fn<Q1_LAYOUT_UNCOMPRESSED>(threads);
-    case Q1_LAYOUT_COMPRESSED: return any_fn<Q1_LAYOUT_COMPRESSED>(threads);
-    default: return any_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(threads);
+    case Q1_LAYOUT_REFERENCE: return outside ? wide_fn<Q1_LAYOUT_REFERENCE>(threads) : any_fn<Q1_LAYOUT_REFERENCE>(threads);
+    case Q1_LAYOUT_UNCOMPRESSED:
+      return outside ? wide_fn<Q1_LAYOUT_UNCOMPRESSED>(threads) : any_fn<Q1_LAYOUT_UNCOMPRESSED>(threads);
+    case Q1_LAYOUT_COMPRESSED:
+      return outside ? wide_fn<Q1_LAYOUT_COMPRESSED>(threads) : any_fn<Q1_LAYOUT_COMPRESSED>(threads);
+    default:
+      return outside ? wide_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(threads) : any_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(threads);
   }
 }
 
@@ -1997,7 +2146,8 @@ uint64_t scan_workspace_bytes(q1_placement placement, const LaunchShape& shape) 
 }
 
 q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const ScanArgs& args,
-                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread) {
+                      const LaunchShape& shape, void* workspace, void* stream, const Spread* spread,
+                      bool table_outside) {
   if (shape.blocks == 0) return Q1_OK;
   const hipStream_t s = (hipStream_t)stream;
   const uint32_t nwaves = shape.threads / 64;
@@ -2029,8 +2179,8 @@ q1_status launch_scan(q1_placement placement, q1_layout layout, bool fast, const
     }
   } else {
     const uint32_t lds = shared_table_bytes(shape.threads) + nwaves * kNumGroups * 8 * 8;
-    hipLaunchKernelGGL(general_kernel(layout, shape.threads), dim3(shape.blocks), dim3(shape.threads), lds, s, a,
-                       shape.tiles_per_thread);
+    hipLaunchKernelGGL(general_kernel(layout, shape.threads, table_outside), dim3(shape.blocks), dim3(shape.threads),
+                       lds, s, a, shape.tiles_per_thread);
   }
   return hip_status(hipGetLastError());
 }

@@ -153,7 +153,8 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
     a.cutoff_day = t0->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t0->filter_cutoff : r.cutoff_day;
     a.out = s.d_out;
     st = q1::launch_zero(s.d_out, 1, s.stream);
-    if (st == Q1_OK) st = q1::launch_scan(Q1_PLACE_IN_REGISTERS, t0->layout, false, a, shape, nullptr, s.stream);
+    if (st == Q1_OK)
+      st = q1::launch_scan(Q1_PLACE_IN_REGISTERS, t0->layout, false, a, shape, nullptr, s.stream, nullptr, true);
     if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,
                                       s.stream) != hipSuccess)
       st = Q1_EDEVICE;

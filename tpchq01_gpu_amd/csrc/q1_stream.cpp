@@ -241,7 +241,7 @@ q1_status q1_run_split(q1_ctx* ctx, const q1_table* t, const q1_columns* host, u
     const auto w0 = std::chrono::steady_clock::now();
     s = launch_zero(ctx->d_aggr, 1, ctx->stream);
     if (s == Q1_OK && hipEventRecord(ctx->ev_begin, ctx->stream) != hipSuccess) s = Q1_EDEVICE;
-    if (s == Q1_OK) s = scan_on(ctx, p, t->layout, t->fast, a, shape, ctx->stream);
+    if (s == Q1_OK) s = scan_on(ctx, p, t->layout, t->fast, a, shape, ctx->stream, !t->fast);
     if (s == Q1_OK && hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess) s = Q1_EDEVICE;
     if (s == Q1_OK &&
         hipMemcpyAsync(ctx->h_aggr, ctx->d_aggr, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost, ctx->stream) !=
