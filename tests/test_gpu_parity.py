@@ -336,15 +336,19 @@ def test_unused_return_flag_code_never_counts(ctx, oracle, pattern):
     bad = rows[rows % 7 == 3] if pattern == "every_7th" else rows[100_000:100_064]
     for i in bad:
         cols["return_flag"][i // 16] |= np.uint32(3 << (2 * (i % 16)))
-    cutoff = 729999
-    want = [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 0, n, cutoff)]
-    # the planted rows passed the filter and no longer count
-    assert sum(g["count"] for g in want) < sum(g["count"] for g in oracle.compressed_kernel(clean, 0, n, cutoff))
     t = ctx.upload(q1.COMPRESSED, cols, n)
-    for p in PLACEMENTS:
-        for launch in (None, q1.make_launch(64, 16), q1.make_launch(256, 256)):
-            note(f"case invalid rf {pattern} placement={p} launch={launch and (launch.threads_per_block, launch.tuples_per_thread)}")
-            assert as_dicts(ctx.run(t, 0, n, cutoff, p, launch)) == want
+    # 727563 + 65535: a compressed cutoff at which every ship date passes, so the kernels
+    # cannot turn the planted rows into filtered ones and test each row's slot instead
+    for cutoff in (729999, 727563 + 65535):
+        want = [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 0, n, cutoff)]
+        # the planted rows passed the filter and no longer count
+        assert sum(g["count"] for g in want) < sum(g["count"] for g in oracle.compressed_kernel(clean, 0, n, cutoff))
+        for p in PLACEMENTS:
+            for launch in (None, q1.make_launch(64, 16), q1.make_launch(256, 256)):
+                note(f"case invalid rf {pattern} cutoff={cutoff} placement={p} "
+                     f"launch={launch and (launch.threads_per_block, launch.tuples_per_thread)}")
+                assert as_dicts(ctx.run(t, 0, n, cutoff, p, launch)) == want
+    cutoff = 729999
     assert as_dicts(ctx.run(t, 5, n - 9, cutoff, q1.PER_THREAD_SHARED_MEM)) == \
         [dict(g, sum_qty=g["sum_qty"] * 100) for g in oracle.compressed_kernel(cols, 5, n - 4, cutoff)]
     t.free()
