@@ -241,9 +241,9 @@ def test_pinned_columns_stream_and_split(ctx):
 
 
 def test_batch_small_grids_span_many_requests(ctx):
-    """Grids of 1, 2 and 3 blocks give every block a share that crosses many request
-    boundaries (256 requests of about 65 tiles, aligned and unaligned starts): every request
-    must still be counted whole, as with the default grid."""
+    """Grids of 1, 2 and 3 blocks make every block work through many requests' blocks,
+    claimed one after another (256 requests of about 65 tiles, aligned and unaligned
+    starts): every request must still be counted whole, as with the default grid."""
     t = ctx.generate(q1.COMPRESSED, SEED + 31, 70_000)
     for rows, begins in ((1040, [0] * 256), (1040, [16 * (i % 7) for i in range(256)]),
                          (1037, [3 + (i % 5) for i in range(256)])):
@@ -257,6 +257,35 @@ def test_batch_small_grids_span_many_requests(ctx):
                                           C.byref(q1.make_launch(0, 0, blocks)), outs), "q1_run_batch")
             for i in range(n):
                 assert q1.result_to_list(outs[i]) == want[i], (rows, blocks, i)
+    t.free()
+
+
+def test_batch_rows_per_lane_and_empty_requests(ctx):
+    """The batched kernel cuts each request into blocks of 256 lanes x tuples_per_thread
+    rows.  Small values give large requests many blocks; empty requests (first, between
+    others and last) own no block, so the search for a block's request must pass over
+    them.  Request sizes straddle the block size; every grid and rows per lane gives the
+    results of the requests run alone.  Values that are not a multiple of 16, or above
+    16384, are refused."""
+    t = ctx.generate(q1.COMPRESSED, SEED + 33, 300_000)
+    spans = [(0, 0), (5, 1), (0, 4095), (7, 4096), (9, 4097), (100, 0), (13, 65_549), (0, 300_000),
+             (299_999, 1), (1234, 0), (50_001, 123_457), (3, 0)]
+    cut = [729999 - 7 * i for i in range(len(spans))]
+    reqs = [q1.Request(t.handle, b, c, cut[i], 0) for i, (b, c) in enumerate(spans)]
+    want = [ctx.run(t, b, c, cut[i], q1.IN_REGISTERS) for i, (b, c) in enumerate(spans)]
+    n = len(reqs)
+    for tpt in (16, 48, 256, 0):
+        for blocks in (1, 5, 0):
+            outs = (q1.Result * n)()
+            note(f"case batch rows per lane tpt={tpt} blocks={blocks}")
+            ctx.check(q1.lib.q1_run_batch(ctx.handle, (q1.Request * n)(*reqs), n,
+                                          C.byref(q1.make_launch(0, tpt, blocks)), outs), "q1_run_batch")
+            for i in range(n):
+                assert q1.result_to_list(outs[i]) == want[i], (tpt, blocks, i)
+    for bad in (24, 16400):
+        outs = (q1.Result * n)()
+        st = q1.lib.q1_run_batch(ctx.handle, (q1.Request * n)(*reqs), n, C.byref(q1.make_launch(0, bad, 0)), outs)
+        assert st == q1.EINVAL, bad
     t.free()
 
 

@@ -1,9 +1,11 @@
 // q1_run_batch: many filter-aggregate requests (uneven row counts, their own cutoffs) in
 // one persistent launch, captured once into a hipGraph and replayed:
-//   H2D copy of the request descriptors (pinned staging) -> zero the per-request tables
-//   (a fill kernel, never a memset node) -> the batched kernel -> D2H copy of the tables.
+//   H2D copy of the request descriptors (pinned staging; it also resets the kernel's claim
+//   counter) -> zero the per-request tables (a fill kernel, never a memset node) -> the
+//   batched kernel -> D2H copy of the tables.
 // The graph is rebuilt only when the capacity bucket, layout, placement or grid changes;
 // each call only rewrites the pinned descriptor buffer before the replay.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <vector>
@@ -36,6 +38,33 @@ void batch_destroy(q1_batch_state* b) {
 }
 
 static uint64_t buf_bytes(uint32_t cap) { return sizeof(BatchHeader) + (uint64_t)cap * sizeof(BatchDesc); }
+
+uint32_t batch_lane_rows(q1_ctx* ctx, q1_layout layout, q1_placement p) {
+  p = batch_placement(p);
+  const uint32_t tuned = ctx->tuning[layout][p].tuples_per_thread;
+  const uint32_t most = batch_max_lane_rows(p);
+  return tuned >= 16 && tuned <= most ? tuned / 16 * 16 : std::min<uint32_t>(256, most);
+}
+
+uint32_t batch_default_blocks(q1_ctx* ctx, q1_layout layout, q1_placement p) {
+  const uint32_t cus = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256);
+  return cus * batch_blocks_per_cu(layout, p);
+}
+
+uint64_t batch_describe(BatchDesc* d, const q1_request& r, uint64_t first, uint32_t lane_rows) {
+  const q1_table* t = r.table;
+  d->cols = t->dev;
+  d->row_begin = r.row_begin;
+  d->row_end = r.row_begin + r.row_count;
+  d->block_begin = first;
+  d->cutoff_day = t->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t->filter_cutoff : r.cutoff_day;
+  d->pad = 0;
+  if (r.row_count == 0) return 0;
+  const uint64_t trows = layout_tile_rows(t->layout);
+  const uint64_t tiles = (d->row_end + trows - 1) / trows - d->row_begin / trows;
+  const uint64_t per_block = (uint64_t)kBatchThreads * (lane_rows / trows);
+  return (tiles + per_block - 1) / per_block;
+}
 
 static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placement, uint32_t cap,
                              uint32_t threads, uint32_t blocks) {
@@ -119,35 +148,36 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
       return Q1_OK;
     }
     std::lock_guard<std::mutex> g(ctx->lock);
-    q1_placement p = ctx->best[layout];
-    if (p != Q1_PLACE_PER_THREAD_SHARED_MEM) p = Q1_PLACE_IN_REGISTERS;
-    // The batched kernel runs 256-thread blocks; only the grid size is tunable.
-    const uint32_t threads = 256;
+    const q1_placement p = batch_placement(ctx->best[layout]);
+    // The batched kernel runs 256-thread blocks; the grid and rows per lane are tunable.
+    const uint32_t threads = kBatchThreads;
     if (launch && launch->threads_per_block && launch->threads_per_block != threads)
       return fail(ctx, Q1_EINVAL, "the batched kernel runs 256 threads per block");
-    const uint32_t blocks = launch && launch->num_blocks ? launch->num_blocks
-                                                         : (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 8;
+    uint32_t lane_rows = batch_lane_rows(ctx, layout, p);
+    if (launch && launch->tuples_per_thread) {
+      lane_rows = launch->tuples_per_thread;
+      if (lane_rows % 16 != 0 || lane_rows > batch_max_lane_rows(p))
+        return fail(ctx, Q1_EINVAL,
+                    "batched tuples_per_thread must be a multiple of 16, at most 256 (shared placement) or 16384");
+    }
+    const uint32_t blocks =
+        launch && launch->num_blocks ? launch->num_blocks : batch_default_blocks(ctx, layout, p);
     uint32_t cap = 16;
     while (cap < n) cap <<= 1;
     q1_status s = build_state(ctx, layout, p, cap, threads, blocks);
     if (s != Q1_OK) return s;
     q1_batch_state* b = ctx->batch;
-    const uint64_t trows = layout_tile_rows(layout);
     BatchDesc* d = reinterpret_cast<BatchDesc*>(b->h_buf + 1);
-    uint64_t tiles = 0, rows = 0;
+    uint64_t entries = 0, rows = 0;
     for (uint32_t i = 0; i < n; i++) {
-      const q1_table* t = reqs[i].table;
-      d[i].cols = t->dev;
-      d[i].row_begin = reqs[i].row_begin;
-      d[i].row_end = reqs[i].row_begin + reqs[i].row_count;
-      d[i].tile_begin = tiles;
-      d[i].cutoff_day = t->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t->filter_cutoff : reqs[i].cutoff_day;
-      d[i].pad = 0;
-      if (reqs[i].row_count) tiles += (d[i].row_end + trows - 1) / trows - d[i].row_begin / trows;
+      entries += batch_describe(&d[i], reqs[i], entries, lane_rows);
       rows += reqs[i].row_count;
     }
+    if (entries >= (1ull << 31)) return fail(ctx, Q1_EINVAL, "batch too large (2^31 blocks of rows)");
     b->h_buf->n = n;
-    b->h_buf->total_tiles = tiles;
+    b->h_buf->total_blocks = entries;
+    b->h_buf->tiles_per_thread = lane_rows / layout_tile_rows(layout);
+    b->h_buf->claimed = 0;
     const auto w0 = std::chrono::steady_clock::now();
     if (hipEventRecord(ctx->ev_begin, ctx->stream) != hipSuccess ||
         hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess ||

@@ -63,5 +63,12 @@ q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
 q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
                   const LaunchShape& shape, void* stream, bool table_outside = false);
 void batch_destroy(q1_batch_state* b);
+// The batched kernel's plan, shared by q1_run_batch and the server: rows per lane (the
+// tuned value of the layout and placement, at most batch_max_lane_rows), the default
+// grid (every block resident at once), and descriptor r of a batch, whose blocks start at
+// list entry `first`; batch_describe returns the request's number of blocks.
+uint32_t batch_lane_rows(q1_ctx* ctx, q1_layout layout, q1_placement p);
+uint32_t batch_default_blocks(q1_ctx* ctx, q1_layout layout, q1_placement p);
+uint64_t batch_describe(BatchDesc* d, const q1_request& r, uint64_t first, uint32_t lane_rows);
 void stream_state_release(q1_ctx* ctx);
 }  // namespace q1

@@ -70,25 +70,34 @@ q1_status launch_domain(q1_layout layout, const q1_columns& cols, uint64_t rows,
 q1_status launch_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutoff_day,
                               uint32_t* filter, void* stream);
 
-// Batched scan: one persistent launch over many requests of one layout, TPC-H domain.
-// The device buffer starts with a BatchHeader followed by `n` BatchDesc records; the grid
-// is fixed, each block takes an equal share of the concatenated tile space of all
-// requests (requests in order, tiles of a request in row order) and merges its partial
-// table into outs[r] whenever its share leaves request r.
+// Batched scan: one launch over many requests of one layout, TPC-H domain.  The device
+// buffer starts with a BatchHeader followed by `n` BatchDesc records.  Request r is cut
+// into the blocks launch_scan would run over its rows alone (256 lanes x
+// tiles_per_thread tiles each); the blocks of all requests, in request order, form one
+// list of total_blocks entries.  A resident grid works through the list, claiming
+// entries from `claimed` (0 at launch), and merges each entry into outs[r].
 struct BatchDesc {
   q1_columns cols;
   uint64_t row_begin, row_end;
-  uint64_t tile_begin;        // first tile of this request in the concatenated tile space
+  uint64_t block_begin;       // request's first entry in the block list
   int32_t cutoff_day;
   uint32_t pad;
 };
 struct BatchHeader {
   uint64_t n;                 // requests
-  uint64_t total_tiles;       // tiles of all requests
-  uint64_t pad[6];
+  uint64_t total_blocks;      // entries of the block list (below 2^31)
+  uint32_t tiles_per_thread;  // tiles of layout_tile_rows rows per lane and entry
+  uint32_t claimed;           // entries claimed beyond the first gridDim.x; 0 at launch
+  uint64_t pad[5];
 };
+constexpr uint32_t kBatchThreads = 256;
 uint32_t layout_tile_rows(q1_layout layout);
-q1_status launch_batch(q1_layout layout, q1_placement placement, const BatchHeader* dev_buffer,
+// The placement the batched kernel runs for a requested one (shared or in registers),
+// the largest rows per lane it accepts for it, and how many of its blocks fit on a CU.
+q1_placement batch_placement(q1_placement p);
+uint32_t batch_max_lane_rows(q1_placement p);
+uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p);
+q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* dev_buffer,
                        q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream);
 
 }  // namespace q1

@@ -35,7 +35,7 @@ namespace q1 {
 namespace {
 
 constexpr int kMaxBlock = 256;
-constexpr int kBatchBlock = 256;  // the batched kernel's block size
+constexpr int kBatchBlock = (int)kBatchThreads;  // the batched kernel's block size
 constexpr uint32_t kNoGroup = 7;
 
 // ------------------------------------------------------------------------------------
@@ -1299,9 +1299,11 @@ __device__ __forceinline__ void add_any_group(const TileGroup<Tile, U>& grp, Acc
 // ------------------------------------------------------------------------------------
 // The fast scan kernel: layout L, placement P.
 // ------------------------------------------------------------------------------------
+// One block's share of a range: block `block` of the grid that launch_scan would run over
+// a's rows, merged into *out.  `lds` is the block's dynamic LDS.
 template <int L, int P, int B>
-__global__ void __launch_bounds__(B)
-scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
+__device__ __forceinline__ void scan_block(const ScanArgs& a, uint64_t block, uint32_t host_tiles_per_thread,
+                                           uint64_t* ws, uint64_t* lds, q1_device_aggregates* out) {
   using Tile = typename ScanTileOf<L, P>::T;
   using Acc = typename AccOf<P, B>::A;
   constexpr int TILE = Tile::TILE;
@@ -1309,7 +1311,6 @@ scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
   constexpr int FLUSH = U * TILE >= 16 ? 1 : 16 / (U * TILE);  // groups per 16 rows
   // The host counts tiles of TileOf<L> (tile_rows); a smaller kernel tile splits each.
   constexpr int SUB = TileOf<L>::T::TILE / TILE;
-  extern __shared__ uint64_t lds[];
 
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
   // Block b covers kernel tiles [first_tile + b*B*tpt, +B*tpt): SUB times the host's
@@ -1317,7 +1318,7 @@ scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
   const uint32_t tiles_per_thread = host_tiles_per_thread * SUB;
   const uint64_t first_tile = a.row_begin / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
-  const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
+  const uint64_t base = first_tile + block * B * tiles_per_thread;
 
   Acc acc;
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
@@ -1406,7 +1407,14 @@ scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
   __syncthreads();
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-  block_merge<Acc::kPackedSmall>(tot, red, a.out + (blockIdx.x & a.out_mask), qty_scale);
+  block_merge<Acc::kPackedSmall>(tot, red, out, qty_scale);
+}
+
+template <int L, int P, int B>
+__global__ void __launch_bounds__(B)
+scan_fast(ScanArgs a, uint32_t host_tiles_per_thread, uint64_t* ws) {
+  extern __shared__ uint64_t lds[];
+  scan_block<L, P, B>(a, blockIdx.x, host_tiles_per_thread, ws, lds, a.out + (blockIdx.x & a.out_mask));
 }
 
 // ------------------------------------------------------------------------------------
@@ -1942,90 +1950,59 @@ __global__ void build_filter(const uint16_t* ship, uint64_t rows, int32_t cut, u
 // ------------------------------------------------------------------------------------
 // The batched persistent kernel.
 // ------------------------------------------------------------------------------------
-template <int L, int P>
-__global__ void __launch_bounds__(kMaxBlock)
-batch_scan(const BatchHeader* hdr, q1_device_aggregates* outs) {
-  // 16-row tiles with ordinary loads: the small streaming tiles of the scan kernel's LDS
-  // placement measured slower here, where each block's share spans several requests.
-  using Tile = typename TileOf<L>::T;
-  using Acc = typename AccOf<P, kBatchBlock>::A;
-  constexpr int TILE = Tile::TILE;
-  constexpr int U = Tile::GROUP;                       // tiles loaded together per lane
-  constexpr int FLUSH = U * TILE >= 16 ? 1 : 16 / (U * TILE);  // groups per 16 rows
-  constexpr uint64_t kMaxTilesPerLane = 16384 / TILE;  // keeps lane partials exact
-  extern __shared__ uint64_t lds[];
-  __shared__ uint32_t first_req;
-
-  const uint64_t n = hdr->n, total = hdr->total_tiles;
-  const BatchDesc* descs = reinterpret_cast<const BatchDesc*>(hdr + 1);
-  const uint64_t span = (total + gridDim.x - 1) / gridDim.x;
-  uint64_t pos = (uint64_t)blockIdx.x * span;
-  const uint64_t stop = pos + span < total ? pos + span : total;
-  if (pos >= stop) return;  // uniform for the whole block
-  if (threadIdx.x == 0) {
-    // last request whose first tile is <= pos
-    uint64_t lo = 0, hi = n;
-    while (hi - lo > 1) {
-      const uint64_t mid = (lo + hi) / 2;
-      if (descs[mid].tile_begin <= pos) lo = mid; else hi = mid;
-    }
-    first_req = (uint32_t)lo;
+// The last request whose first block is <= v, searching forward from request r (whose
+// first block is <= v).  First blocks never decrease along the list, so the lanes that
+// pass form a prefix of the wavefront; every wavefront of the block computes the same r.
+__device__ __forceinline__ uint32_t find_request(const BatchDesc* descs, uint32_t n, uint32_t r, uint64_t v) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (;;) {
+    const uint32_t i = r + 1 + lane;
+    const uint64_t m = ballot(i < n && descs[i].block_begin <= v);
+    r += (uint32_t)__popcll(m);
+    if (m != ~0ull) return r;
   }
-  __syncthreads();
-  uint64_t r = first_req;
-  uint64_t* red = lds + Acc::lds_bytes(blockDim.x) / 8;
-  while (pos < stop && r < n) {
+}
+
+// The batched kernel: the requests' scan blocks (request r's block k is the block k that
+// launch_scan would run over r's rows alone, with hdr->tiles_per_thread) form one list of
+// hdr->total_blocks blocks.  A resident grid works through the list: block b first runs
+// list entry b, then claims entries gridDim.x, gridDim.x + 1, ... from hdr->claimed in
+// the order blocks finish, so the blocks in flight read neighbouring stretches of HBM
+// whatever the request sizes, and the load balances to one entry.  Each entry runs the
+// scan kernel's block body and merges into its request's table.  Every block exits once
+// the list is used up; hdr->claimed must be 0 at launch.
+// The claim counter comes as its own pointer and the header and descriptors as read-only
+// ones, so the descriptors load through the scalar cache into scalar registers (a kernel
+// that might write them reads them with vector loads, into vector registers).
+template <int L, int P>
+__global__ void __launch_bounds__(kBatchBlock)
+batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ descs, uint32_t* __restrict__ claimed,
+           q1_device_aggregates* outs) {
+  extern __shared__ uint64_t lds[];
+  __shared__ uint32_t next_entry;
+  const uint32_t n = (uint32_t)hdr->n, tpt = hdr->tiles_per_thread;
+  const uint64_t total = hdr->total_blocks;
+  uint64_t v = blockIdx.x;
+  uint32_t r = 0;
+  while (v < total) {
+    uint32_t claim = 0;
+    if (threadIdx.x == 0) claim = gridDim.x + atomicAdd(claimed, 1u);
+    r = find_request(descs, n, r, v);
     const BatchDesc d = descs[r];
-    const uint64_t first_tile = d.row_begin / TILE;
-    const uint64_t req_tiles = (d.row_end + TILE - 1) / TILE - first_tile;
-    const uint64_t req_end = d.tile_begin + req_tiles;
-    if (req_tiles == 0 || pos >= req_end) {  // empty request or already past it
-      r++;
-      continue;
-    }
-    uint64_t seg_end = stop < req_end ? stop : req_end;
-    const uint64_t cap = pos + (uint64_t)blockDim.x * kMaxTilesPerLane;
-    if (seg_end > cap) seg_end = cap;
-    const int32_t cut = column_cutoff<L>(d.cutoff_day);
-    Acc acc;
-    acc.init(reinterpret_cast<char*>(lds));
-    // tile g of the concatenated space is tile first_tile + (g - tile_begin) of the request
-    const uint64_t seg_end_tile = first_tile + (seg_end - d.tile_begin);
-    uint32_t n = 0;
-    for (uint64_t b = pos; b < seg_end; b += (uint64_t)U * kBatchBlock, n++) {
-      TileGroup<Tile, U> grp;
-      grp.load(d.cols, first_tile + (b + threadIdx.x - d.tile_begin), kBatchBlock, U, seg_end_tile, d.row_begin,
-               d.row_end);
-      if (grp.plain) {
-#pragma unroll
-        for (int m = 0; m < U; m++)
-#pragma unroll
-          for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
-      } else {
-#pragma unroll
-        for (int m = 0; m < U; m++) {
-          if (!grp.use[m]) continue;
-          if (!grp.edge[m]) {
-#pragma unroll
-            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<false>(j, cut, 0));
-          } else {
-#pragma unroll
-            for (int j = 0; j < TILE; j++) acc.add(grp.tile[m].template fast<true>(j, cut, grp.valid[m]));
-          }
-        }
-      }
-      if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
-    }
-    acc.flush();
-    LaneTotals tot;
-    acc.totals(tot, reinterpret_cast<char*>(lds));
+    ScanArgs a;
+    a.cols = d.cols;
+    a.row_begin = d.row_begin;
+    a.row_end = d.row_end;
+    a.cutoff_day = d.cutoff_day;
+    a.out = outs + r;
+    a.out_mask = 0;
+    scan_block<L, P, kBatchBlock>(a, v - d.block_begin, tpt, nullptr, lds, outs + r);
+    if (threadIdx.x == 0) next_entry = claim;
+    // Also orders this entry's reads of the merge scratch before the next entry's writes.
     __syncthreads();
-    const uint64_t qty_scale =
-        (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-    block_merge<Acc::kPackedSmall>(tot, red, &outs[r], qty_scale);
-    __syncthreads();  // red and the placement's LDS table are reused by the next segment
-    pos = seg_end;
-    if (pos >= req_end) r++;
+    // The same value in every lane: readfirstlane keeps the entry's bounds in scalar
+    // registers (an LDS load alone counts as divergent, moving them to vector registers).
+    v = (uint32_t)__builtin_amdgcn_readfirstlane((int)next_entry);
   }
 }
 
@@ -2236,27 +2213,57 @@ q1_status launch_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutof
 
 uint32_t layout_tile_rows(q1_layout layout) { return tile_rows(layout); }
 
+// The batched kernel's variants: the shared placement runs the narrow per-lane table (the
+// packed one for the compressed layouts), as launch_scan does for at most 256 rows per lane.
 template <int L>
 static void* batch_fn(q1_placement p) {
-  return p == Q1_PLACE_PER_THREAD_SHARED_MEM ? (void*)batch_scan<L, Q1_PLACE_PER_THREAD_SHARED_MEM>
-                                             : (void*)batch_scan<L, Q1_PLACE_IN_REGISTERS>;
+  constexpr bool units = L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN;
+  if (p != Q1_PLACE_PER_THREAD_SHARED_MEM) return (void*)batch_scan<L, Q1_PLACE_IN_REGISTERS>;
+  return units ? (void*)batch_scan<L, kPlaceSharedPacked> : (void*)batch_scan<L, kPlaceSharedNarrow>;
 }
 
-q1_status launch_batch(q1_layout layout, q1_placement placement, const BatchHeader* buf,
-                       q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream) {
-  if (placement != Q1_PLACE_PER_THREAD_SHARED_MEM) placement = Q1_PLACE_IN_REGISTERS;
-  if (threads != (uint32_t)kBatchBlock) return Q1_EINVAL;
-  uint32_t lds = (placement == Q1_PLACE_PER_THREAD_SHARED_MEM) ? AccShared<kBatchBlock>::lds_bytes(threads) : 0;
-  lds += threads / 64 * kNumGroups * NF * 8;
-  void* fn;
+static void* batch_kernel(q1_layout layout, q1_placement p) {
   switch (layout) {
-    case Q1_LAYOUT_REFERENCE: fn = batch_fn<Q1_LAYOUT_REFERENCE>(placement); break;
-    case Q1_LAYOUT_UNCOMPRESSED: fn = batch_fn<Q1_LAYOUT_UNCOMPRESSED>(placement); break;
-    case Q1_LAYOUT_COMPRESSED: fn = batch_fn<Q1_LAYOUT_COMPRESSED>(placement); break;
-    default: fn = batch_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(placement); break;
+    case Q1_LAYOUT_REFERENCE: return batch_fn<Q1_LAYOUT_REFERENCE>(p);
+    case Q1_LAYOUT_UNCOMPRESSED: return batch_fn<Q1_LAYOUT_UNCOMPRESSED>(p);
+    case Q1_LAYOUT_COMPRESSED: return batch_fn<Q1_LAYOUT_COMPRESSED>(p);
+    default: return batch_fn<Q1_LAYOUT_COMPRESSED_PUSHDOWN>(p);
   }
-  void* args[] = {(void*)&buf, (void*)&outs};
-  return hip_status(hipLaunchKernel(fn, dim3(blocks), dim3(threads), args, lds, (hipStream_t)stream));
+}
+
+static uint32_t batch_lds(q1_layout layout, q1_placement p) {
+  const bool units = layout == Q1_LAYOUT_COMPRESSED || layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN;
+  uint32_t lds = p != Q1_PLACE_PER_THREAD_SHARED_MEM ? 0
+                 : units                            ? packed_table_bytes(kBatchBlock)
+                                                    : shared_table_bytes(kBatchBlock);
+  return lds + kBatchBlock / 64 * kNumGroups * NF * 8;
+}
+
+q1_placement batch_placement(q1_placement p) {
+  return p == Q1_PLACE_PER_THREAD_SHARED_MEM ? p : Q1_PLACE_IN_REGISTERS;
+}
+
+uint32_t batch_max_lane_rows(q1_placement p) { return batch_placement(p) == Q1_PLACE_PER_THREAD_SHARED_MEM ? 256 : 16384; }
+
+uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p) {
+  p = batch_placement(p);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)batch_kernel(layout, p), kBatchBlock,
+                                                   batch_lds(layout, p)) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (uint32_t)per_cu;
+}
+
+q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* buf, q1_device_aggregates* outs,
+                       uint32_t blocks, uint32_t threads, void* stream) {
+  placement = batch_placement(placement);
+  if (threads != (uint32_t)kBatchBlock || blocks == 0) return Q1_EINVAL;
+  const BatchDesc* descs = reinterpret_cast<const BatchDesc*>(buf + 1);
+  uint32_t* claimed = &buf->claimed;
+  void* args[] = {(void*)&buf, (void*)&descs, (void*)&claimed, (void*)&outs};
+  return hip_status(hipLaunchKernel(batch_kernel(layout, placement), dim3(blocks), dim3(threads), args,
+                                    batch_lds(layout, placement), (hipStream_t)stream));
 }
 
 }  // namespace q1

@@ -65,7 +65,6 @@ struct q1_server {
   q1_ctx* ctx = nullptr;
   q1_server_config cfg{};
   q1_placement placement = Q1_PLACE_IN_REGISTERS;
-  uint32_t blocks = 0;
   std::mutex m;
   std::condition_variable cv_sched, cv_done, cv_slot;
   std::deque<Pending> queue;
@@ -108,11 +107,13 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
     return Q1_ENOMEM;
   }
   std::memset(s.h_buf, 0, bb);
+  const uint32_t blocks = q1::batch_default_blocks(sv->ctx, layout, sv->placement);
   q1_status st = Q1_OK;
   if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) st = Q1_EDEVICE;
   if (st == Q1_OK && hipMemcpyAsync(s.d_buf, s.h_buf, bb, hipMemcpyHostToDevice, s.stream) != hipSuccess) st = Q1_EDEVICE;
   if (st == Q1_OK) st = q1::launch_zero(s.d_out, cap, s.stream);
-  if (st == Q1_OK) st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, sv->blocks, 256, s.stream);
+  if (st == Q1_OK)
+    st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, blocks, q1::kBatchThreads, s.stream);
   if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
     st = Q1_EDEVICE;
   hipGraph_t g = nullptr;
@@ -165,21 +166,15 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
   while (cap < n) cap <<= 1;
   q1_status st = prepare_slot(sv, s, cap, t0->layout);
   if (st != Q1_OK) return st;
-  const uint64_t trows = q1::layout_tile_rows(t0->layout);
+  const uint32_t lane_rows = q1::batch_lane_rows(sv->ctx, t0->layout, sv->placement);
   q1::BatchDesc* d = reinterpret_cast<q1::BatchDesc*>(s.h_buf + 1);
-  uint64_t tiles = 0;
-  for (uint32_t i = 0; i < n; i++) {
-    const q1_request& r = s.batch[i].req;
-    d[i].cols = r.table->dev;
-    d[i].row_begin = r.row_begin;
-    d[i].row_end = r.row_begin + r.row_count;
-    d[i].tile_begin = tiles;
-    d[i].cutoff_day = r.table->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? r.table->filter_cutoff : r.cutoff_day;
-    d[i].pad = 0;
-    if (r.row_count) tiles += (d[i].row_end + trows - 1) / trows - d[i].row_begin / trows;
-  }
+  uint64_t entries = 0;
+  for (uint32_t i = 0; i < n; i++) entries += q1::batch_describe(&d[i], s.batch[i].req, entries, lane_rows);
+  if (entries >= (1ull << 31)) return Q1_EINVAL;
   s.h_buf->n = n;
-  s.h_buf->total_tiles = tiles;
+  s.h_buf->total_blocks = entries;
+  s.h_buf->tiles_per_thread = lane_rows / q1::layout_tile_rows(t0->layout);
+  s.h_buf->claimed = 0;
   if (hipGraphLaunch(s.exec, s.stream) != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess)
     return Q1_EDEVICE;
   return Q1_OK;
@@ -323,7 +318,6 @@ q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server**
     sv->cfg = c;
     const q1_placement best = ctx->best[Q1_LAYOUT_COMPRESSED];
     sv->placement = best == Q1_PLACE_PER_THREAD_SHARED_MEM ? best : Q1_PLACE_IN_REGISTERS;
-    sv->blocks = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256) * 8;  // as q1_run_batch
     sv->slots.resize(c.max_inflight);
     (void)hipSetDevice(ctx->device);
     for (auto& s : sv->slots) {
