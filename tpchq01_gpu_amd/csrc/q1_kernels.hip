@@ -1950,6 +1950,31 @@ __global__ void build_filter(const uint16_t* ship, uint64_t rows, int32_t cut, u
 // ------------------------------------------------------------------------------------
 // The batched persistent kernel.
 // ------------------------------------------------------------------------------------
+// Column pointers read from a descriptor.  Pointers that arrive as kernel arguments are
+// known to be global; generic ones loaded from memory make flat loads: 64-bit vector
+// addresses, and flat loads count against the LDS wait counter too, so every wait for an
+// LDS atomic would also wait for them.  Reading the pointers as global-qualified values
+// keeps the tile loads global (a scalar base plus a vector offset).
+typedef const __attribute__((address_space(1))) void* GlobalPtr;
+struct GlobalColumns {
+  GlobalPtr ship_date, quantity, extended_price, discount, tax, return_flag, line_status, filter;
+};
+static_assert(sizeof(GlobalColumns) == sizeof(q1_columns), "same layout as q1_columns");
+
+__device__ __forceinline__ q1_columns load_global_columns(const q1_columns* src) {
+  const GlobalColumns g = *reinterpret_cast<const GlobalColumns*>(src);
+  q1_columns c;
+  c.ship_date = (const void*)g.ship_date;
+  c.quantity = (const void*)g.quantity;
+  c.extended_price = (const void*)g.extended_price;
+  c.discount = (const void*)g.discount;
+  c.tax = (const void*)g.tax;
+  c.return_flag = (const void*)g.return_flag;
+  c.line_status = (const void*)g.line_status;
+  c.filter = (const void*)g.filter;
+  return c;
+}
+
 // The last request whose first block is <= v, searching forward from request r (whose
 // first block is <= v).  First blocks never decrease along the list, so the lanes that
 // pass form a prefix of the wavefront; every wavefront of the block computes the same r.
@@ -1990,7 +2015,7 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
     r = find_request(descs, n, r, v);
     const BatchDesc d = descs[r];
     ScanArgs a;
-    a.cols = d.cols;
+    a.cols = load_global_columns(&descs[r].cols);
     a.row_begin = d.row_begin;
     a.row_end = d.row_end;
     a.cutoff_day = d.cutoff_day;
