@@ -1978,13 +1978,21 @@ __device__ __forceinline__ q1_columns load_global_columns(const q1_columns* src)
 // The last request whose first block is <= v, searching forward from request r (whose
 // first block is <= v).  First blocks never decrease along the list, so the lanes that
 // pass form a prefix of the wavefront; every wavefront of the block computes the same r.
-__device__ __forceinline__ uint32_t find_request(const BatchDesc* descs, uint32_t n, uint32_t r, uint64_t v) {
+// next_begin receives the first block of the request after it (total past the last), so
+// later entries below it need no search.
+__device__ __forceinline__ uint32_t find_request(const BatchDesc* descs, uint32_t n, uint32_t r, uint64_t v,
+                                                 uint64_t total, uint64_t& next_begin) {
   const uint32_t lane = threadIdx.x & 63;
   for (;;) {
     const uint32_t i = r + 1 + lane;
-    const uint64_t m = ballot(i < n && descs[i].block_begin <= v);
-    r += (uint32_t)__popcll(m);
-    if (m != ~0ull) return r;
+    const uint64_t b = i < n ? descs[i].block_begin : total;  // total > v
+    const uint32_t pass = (uint32_t)__popcll(ballot(b <= v));
+    r += pass;
+    if (pass < 64) {
+      next_begin = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, (int)pass) |
+                   (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), (int)pass) << 32;
+      return r;
+    }
   }
 }
 
@@ -2007,13 +2015,19 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
   __shared__ uint32_t next_entry;
   const uint32_t n = (uint32_t)hdr->n, tpt = hdr->tiles_per_thread;
   const uint64_t total = hdr->total_blocks;
-  uint64_t v = blockIdx.x;
+  uint64_t v = blockIdx.x, next_begin = 0;
   uint32_t r = 0;
   while (v < total) {
-    uint32_t claim = 0;
-    if (threadIdx.x == 0) claim = gridDim.x + atomicAdd(claimed, 1u);
-    r = find_request(descs, n, r, v);
+    if (v >= next_begin) r = find_request(descs, n, r, v, total, next_begin);
     const BatchDesc d = descs[r];
+    // The next entry is claimed after the search's load and before the tile loads: loads
+    // retire in issue order, so a claim issued before the search would make its wait
+    // longer, while the claim's L2 round trip ends before the first tiles arrive from HBM.
+    // atomicInc (no wrap below 2^32 - 1) rather than atomicAdd: the compiler turns a
+    // uniform-address atomicAdd into a wavefront-aggregated one that waits for the
+    // returned value at once; this value is needed only at the entry's end.
+    uint32_t claim = 0;
+    if (threadIdx.x == 0) claim = atomicInc(claimed, 0xFFFFFFFFu);
     ScanArgs a;
     a.cols = load_global_columns(&descs[r].cols);
     a.row_begin = d.row_begin;
@@ -2022,7 +2036,7 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
     a.out = outs + r;
     a.out_mask = 0;
     scan_block<L, P, kBatchBlock>(a, v - d.block_begin, tpt, nullptr, lds, outs + r);
-    if (threadIdx.x == 0) next_entry = claim;
+    if (threadIdx.x == 0) next_entry = gridDim.x + claim;
     // Also orders this entry's reads of the merge scratch before the next entry's writes.
     __syncthreads();
     // The same value in every lane: readfirstlane keeps the entry's bounds in scalar
