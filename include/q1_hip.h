@@ -115,7 +115,9 @@ typedef struct {
 } q1_device_aggregates;
 
 /* Launch shape (the reference's --threads-per-block / --tuples-per-thread,
- * src/main.cu:255-360, 627-650).  Zero fields mean "from the autotuning table". */
+ * src/main.cu:255-360, 627-650).  Zero fields mean "from the autotuning table"; launches
+ * of q1_enqueue and q1_run_streaming without Q1_LAUNCH_ASSUME_TPCH_DOMAIN run the
+ * any-value kernel, whose zero fields mean 256 threads and 1024 tuples per thread. */
 typedef struct {
   uint32_t threads_per_block; /* 64, 128 or 256 (the batched kernel: 256) */
   uint32_t tuples_per_thread; /* multiple of 16 */

@@ -47,10 +47,13 @@ q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p) {
 }
 
 void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows, const q1_launch* in,
-                    q1_launch* out) {
+                    q1_launch* out, bool any_value_kernel) {
   const q1_launch& t = ctx->tuning[layout][p];
-  out->threads_per_block = in && in->threads_per_block ? in->threads_per_block : t.threads_per_block;
-  out->tuples_per_thread = in && in->tuples_per_thread ? in->tuples_per_thread : t.tuples_per_thread;
+  const bool any_value = any_value_kernel && !(in && (in->flags & Q1_LAUNCH_ASSUME_TPCH_DOMAIN));
+  const uint32_t tuned_threads = any_value ? 256 : t.threads_per_block;
+  const uint32_t tuned_tuples = any_value ? 1024 : t.tuples_per_thread;
+  out->threads_per_block = in && in->threads_per_block ? in->threads_per_block : tuned_threads;
+  out->tuples_per_thread = in && in->tuples_per_thread ? in->tuples_per_thread : tuned_tuples;
   out->num_blocks = in ? in->num_blocks : 0;
   out->flags = in ? in->flags : 0;
   // The tuned tuples_per_thread is measured on large tables; a smaller range halves it
@@ -649,7 +652,7 @@ q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout, cons
   }
   const q1_placement p = resolve_placement(ctx, layout, placement);
   q1_launch l;
-  resolve_launch(ctx, layout, p, num_tuples, launch, &l);
+  resolve_launch(ctx, layout, p, num_tuples, launch, &l, true);
   LaunchShape shape;
   const char* why = "";
   q1_status s = make_shape(p, layout, &l, row_begin, row_begin + num_tuples, &shape, &why);

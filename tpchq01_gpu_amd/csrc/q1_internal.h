@@ -54,8 +54,12 @@ namespace q1 {
 q1_status fail(q1_ctx* ctx, q1_status s, const std::string& why);
 q1_status hip_fail(q1_ctx* ctx, hipError_t e, const char* what);
 q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p);
+// Fills a launch from the caller's, with the tuned values where it leaves them 0.  Launches
+// without Q1_LAUNCH_ASSUME_TPCH_DOMAIN that run the any-value kernel (q1_enqueue,
+// q1_run_streaming) take that kernel's own defaults instead: it runs every placement
+// alike, and is fastest with 256 threads and 1024 rows per lane.
 void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows, const q1_launch* in,
-                    q1_launch* out);
+                    q1_launch* out, bool any_value_kernel = false);
 uint64_t algorithmic_bytes(q1_layout layout, uint64_t rows);
 q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes);
 // launch_scan with the context's spread tables when `stream` is the context stream.

@@ -107,7 +107,7 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
     std::lock_guard<std::mutex> g(ctx->lock);
     const q1_placement p = resolve_placement(ctx, layout, placement);
     q1_launch l;
-    resolve_launch(ctx, layout, p, chunk_rows, launch, &l);
+    resolve_launch(ctx, layout, p, chunk_rows, launch, &l, true);
     StreamState* st = nullptr;
     q1_status s = prepare(ctx, layout, chunk_rows, num_streams, &st);
     if (s != Q1_OK) return s;
