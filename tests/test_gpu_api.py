@@ -196,21 +196,23 @@ def test_batch_pushdown_and_mixed_layouts(ctx, oracle):
 @pytest.mark.parametrize("layout", [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED])
 def test_batch_every_layout_and_batch_placement(layout):
     """The batched kernel for each layout with both of its placements (the register table,
-    and the LDS table that reads the columns in small streaming tiles), on ragged ranges:
-    every result equals the request run alone."""
+    and the LDS table that reads the columns in small streaming tiles), on ragged ranges,
+    at the tuned rows per lane and at 16 (many blocks per request): every result equals
+    the request run alone."""
     rng = np.random.default_rng(7 + layout)
     with q1.Context(0) as c:
         tables = [c.generate(layout, SEED + 11 + k, rows) for k, rows in enumerate([1001, 70_003, 1_000_001])]
         for p in (q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM):
             c.set_best_placement(layout, p)
-            for n in (1, 33, 200):
+            for n, tpt in ((1, 0), (33, 0), (200, 0), (60, 16)):
                 reqs, meta = make_requests(c, tables, rng, n)
                 outs = (q1.Result * n)()
-                note(f"case batch layout={q1.LAYOUT_NAMES[layout]} placement={p} n={n}")
-                c.check(q1.lib.q1_run_batch(c.handle, (q1.Request * n)(*reqs), n, None, outs), "q1_run_batch")
+                note(f"case batch layout={q1.LAYOUT_NAMES[layout]} placement={p} n={n} tpt={tpt}")
+                c.check(q1.lib.q1_run_batch(c.handle, (q1.Request * n)(*reqs), n,
+                                            C.byref(q1.make_launch(0, tpt, 0)), outs), "q1_run_batch")
                 assert c.last_timing().placement == p
                 for i, (t, b, s, cut) in enumerate(meta):
-                    assert q1.result_to_list(outs[i]) == c.run(t, b, s, cut, q1.IN_REGISTERS), (p, n, i)
+                    assert q1.result_to_list(outs[i]) == c.run(t, b, s, cut, q1.IN_REGISTERS), (p, n, tpt, i)
         for t in tables:
             t.free()
 
