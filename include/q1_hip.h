@@ -230,7 +230,8 @@ typedef struct {
  * share one launch; any other mix runs request by request.  `launch` may be NULL; its
  * threads_per_block must be 0 or 256; tuples_per_thread (rows per lane of each block of a
  * request; 0 = the tuned value) a multiple of 16, at most 256 for the shared placement and
- * 16384 for the register one; num_blocks 0 = as many blocks as are resident at once. */
+ * 16384 for the register one; num_blocks 0 = as many blocks as are resident at once
+ * (at most 2^24). */
 q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t n, const q1_launch* launch,
                        q1_result* outs);
 

@@ -162,6 +162,8 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     }
     const uint32_t blocks =
         launch && launch->num_blocks ? launch->num_blocks : batch_default_blocks(ctx, layout, p);
+    // The kernel's claims count in 32 bits from gridDim.x up to the number of blocks of rows.
+    if (blocks > (1u << 24)) return fail(ctx, Q1_EINVAL, "batched num_blocks must be at most 2^24");
     uint32_t cap = 16;
     while (cap < n) cap <<= 1;
     q1_status s = build_state(ctx, layout, p, cap, threads, blocks);
