@@ -41,6 +41,31 @@ class OCompressed(C.Structure):
                 ["shipdate", "quantity", "extendedprice", "discount", "tax", "returnflag", "linestatus"]]
 
 
+COMMON_FLAGS = "-O3 -ftree-vectorize -funroll-all-loops -fomit-frame-pointer"
+# ISA-level builds for the timed CPU baseline, best first, with the CPU features each needs.
+ISA_BUILDS = [("x86-64-v4", {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}),
+              ("x86-64-v3", {"avx2", "bmi2", "fma", "movbe", "f16c"})]
+
+
+def baseline_build() -> tuple[str, str]:
+    """The oracle build for timing on this host: the highest ISA level whose features
+    /proc/cpuinfo lists (the reference builds with -march=native), and its compiler flags."""
+    flags = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+    except OSError:
+        pass
+    for name, need in ISA_BUILDS:
+        path = os.path.join(BUILD, f"liboracle_{name}.so")
+        if need <= flags and os.path.exists(path):
+            return path, f"gcc {COMMON_FLAGS} -march={name}"
+    return os.path.join(BUILD, "liboracle.so"), f"gcc {COMMON_FLAGS}"
+
+
 def build(mutant: int | None = None) -> str:
     """Builds the oracle library (or a planted-mistake variant) and returns its path."""
     if mutant is None:
@@ -66,6 +91,8 @@ class Oracle:
             getattr(lib, name).argtypes = [C.POINTER(OLineitem), I64, I64, I32, C.POINTER(OResult)]
         lib.q1o_hyper.restype = C.c_int
         lib.q1o_hyper.argtypes = [C.POINTER(OLineitem), I64, I64, I32, C.POINTER(OResult)]
+        lib.q1o_hyper_parallel.restype = C.c_int
+        lib.q1o_hyper_parallel.argtypes = [C.POINTER(OLineitem), I64, I64, I32, C.c_int, I64, C.POINTER(OResult)]
         lib.q1o_naive_parallel.restype = None
         lib.q1o_naive_parallel.argtypes = [C.POINTER(OLineitem), I64, I64, I32, C.c_int, I64, C.POINTER(OResult)]
         lib.q1o_compress.restype = None
@@ -76,6 +103,9 @@ class Oracle:
         lib.q1o_get_line_status.argtypes = [P, I64]
         lib.q1o_compressed_kernel.restype = None
         lib.q1o_compressed_kernel.argtypes = [C.POINTER(OCompressed), I64, I64, I32, C.POINTER(OResult)]
+        lib.q1o_compressed_parallel.restype = None
+        lib.q1o_compressed_parallel.argtypes = [C.POINTER(OCompressed), I64, I64, I32, C.c_int, I64,
+                                                C.POINTER(OResult)]
         lib.q1o_precompute_filter.restype = None
         lib.q1o_precompute_filter.argtypes = [P, I64, I32, P]
         lib.q1o_pushdown_kernel.restype = None
@@ -147,6 +177,14 @@ class Oracle:
         flag = self.lib.q1o_hyper(C.byref(self._li(ref_cols)), begin, end, cutoff, C.byref(r))
         return self.to_list(r), int(flag)
 
+    def hyper_parallel(self, ref_cols: dict, begin: int, end: int, cutoff: int, threads: int,
+                       morsel_rows: int = 10 * 1024):
+        self._check_ref(ref_cols)
+        r = OResult()
+        flag = self.lib.q1o_hyper_parallel(C.byref(self._li(ref_cols)), begin, end, cutoff, threads, morsel_rows,
+                                           C.byref(r))
+        return self.to_list(r), int(flag)
+
     def naive_parallel(self, ref_cols: dict, begin: int, end: int, cutoff: int, threads: int,
                        morsel_rows: int = 10 * 1024):
         self._check_ref(ref_cols)
@@ -178,6 +216,13 @@ class Oracle:
     def compressed_kernel(self, ccols: dict, begin: int, end: int, cutoff: int):
         r = OResult()
         self.lib.q1o_compressed_kernel(C.byref(self._cc(ccols)), begin, end, cutoff, C.byref(r))
+        return self.to_list(r)
+
+    def compressed_parallel(self, ccols: dict, begin: int, end: int, cutoff: int, threads: int,
+                            morsel_rows: int = 1 << 20):
+        r = OResult()
+        self.lib.q1o_compressed_parallel(C.byref(self._cc(ccols)), begin, end, cutoff, threads, morsel_rows,
+                                         C.byref(r))
         return self.to_list(r)
 
     def precompute_filter(self, ship_u16: np.ndarray, n: int, cutoff: int) -> np.ndarray:

@@ -13,6 +13,7 @@
  *  10 last filter word dropped                    11 sum_disc accumulates (1 - disc)
  *  12 X100: the last partial vector of a morsel is dropped
  *  13 AVX-512: the last of the 8 table lanes is left out of the final sum
+ *  14 compressed morsel-parallel: a range's last partial morsel is dropped
  */
 #include "q1_oracle.h"
 
@@ -150,10 +151,10 @@ void q1o_naive(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutof
 }
 
 /* ---- KernelHyPer<true> (cpu/kernel_hyper.hpp:73-118) ---------------------------------- */
-int q1o_hyper(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff,
-              q1o_result* out) {
-  acc_t a[6];
-  memset(a, 0, sizeof(a));
+/* Rows [begin, end) added into `a` with every operation overflow-checked; returns 1 if a
+ * check fired. */
+static int hyper_range(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff, acc_t* a,
+                       int64_t* unexpected_out) {
   int64_t unexpected = 0;
   int flag = 0;
   for (int64_t i = begin; i < end; i++) {
@@ -180,7 +181,84 @@ int q1o_hyper(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff
       a[s].count = (int64_t)c;
     }
   }
+  *unexpected_out += unexpected;
+  return flag;
+}
+
+int q1o_hyper(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff,
+              q1o_result* out) {
+  acc_t a[6];
+  memset(a, 0, sizeof(a));
+  int64_t unexpected = 0;
+  const int flag = hyper_range(li, begin, end, cutoff, a, &unexpected);
   acc_to_result(a, unexpected, out);
+  return flag;
+}
+
+/* Morsel-parallel KernelHyPer: workers claim morsels with fetch_add (the pattern of
+ * cpu/kernel_x100.hpp:456-500), each keeps a private overflow-checked table, and the
+ * tables are merged with overflow-checked adds. */
+typedef struct {
+  const q1o_lineitem* li;
+  int64_t end;
+  int32_t cutoff;
+  int64_t morsel;
+  _Atomic int64_t* next;
+  acc_t a[6];
+  int64_t unexpected;
+  int flag;
+} hworker_t;
+
+static void* hyper_worker(void* arg) {
+  hworker_t* w = (hworker_t*)arg;
+  memset(w->a, 0, sizeof(w->a));
+  w->unexpected = 0;
+  w->flag = 0;
+  for (;;) {
+    const int64_t off = atomic_fetch_add(w->next, w->morsel);
+    if (off >= w->end) break;
+    const int64_t stop = off + w->morsel < w->end ? off + w->morsel : w->end;
+    w->flag |= hyper_range(w->li, off, stop, w->cutoff, w->a, &w->unexpected);
+  }
+  return NULL;
+}
+
+int q1o_hyper_parallel(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff, int threads,
+                       int64_t morsel_rows, q1o_result* out) {
+  if (threads < 1) threads = 1;
+  if (morsel_rows < 1) morsel_rows = 10 * 1024;
+  _Atomic int64_t next = begin;
+  hworker_t* ws = (hworker_t*)calloc((size_t)threads, sizeof(hworker_t));
+  pthread_t* ts = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  for (int t = 0; t < threads; t++) {
+    ws[t].li = li; ws[t].end = end; ws[t].cutoff = cutoff; ws[t].morsel = morsel_rows; ws[t].next = &next;
+    pthread_create(&ts[t], NULL, hyper_worker, &ws[t]);
+  }
+  acc_t total[6];
+  memset(total, 0, sizeof(total));
+  int64_t unexpected = 0;
+  int flag = 0;
+  for (int t = 0; t < threads; t++) {
+    pthread_join(ts[t], NULL);
+    flag |= ws[t].flag;
+    for (int g = 0; g < 6; g++) {
+      long long q = total[g].sum_qty, bp = total[g].sum_base_price, d = total[g].sum_disc, c = total[g].count;
+      flag |= __builtin_saddll_overflow(q, ws[t].a[g].sum_qty, &q);
+      flag |= __builtin_saddll_overflow(bp, ws[t].a[g].sum_base_price, &bp);
+      flag |= __builtin_saddll_overflow(d, ws[t].a[g].sum_disc, &d);
+      flag |= __builtin_saddll_overflow(c, ws[t].a[g].count, &c);
+      flag |= __builtin_add_overflow(total[g].sum_disc_price, ws[t].a[g].sum_disc_price, &total[g].sum_disc_price);
+      flag |= __builtin_add_overflow(total[g].sum_charge, ws[t].a[g].sum_charge, &total[g].sum_charge);
+      total[g].sum_qty = q;
+      total[g].sum_base_price = bp;
+      total[g].sum_disc = d;
+      total[g].count = c;
+    }
+    unexpected += ws[t].unexpected;
+  }
+  free(ws);
+  free(ts);
+  acc_to_result(total, unexpected, out);
   return flag;
 }
 
@@ -310,6 +388,63 @@ void q1o_compressed_kernel(const q1o_compressed* c, int64_t begin, int64_t end, 
   for (int64_t i = begin; i < end; i++)
     if (passes(c->shipdate[i], ccut)) aggregate_compressed(a, c, i);
   acc_to_result(a, 0, out);
+}
+
+/* The compressed-kernel semantics morsel-parallel, in the pattern of Morsel::Query
+ * (cpu/kernel_x100.hpp:456-500): workers claim fixed-size morsels of [begin, end) with
+ * fetch_add, aggregate them into private tables, and the tables are added up exactly.
+ * Used to check large device runs (hundreds of millions of rows) in seconds. */
+typedef struct {
+  const q1o_compressed* c;
+  int64_t end, ccut, morsel;
+  _Atomic int64_t* next;
+  acc_t a[6];
+} cworker_t;
+
+static void* compressed_worker(void* arg) {
+  cworker_t* w = (cworker_t*)arg;
+  memset(w->a, 0, sizeof(w->a));
+  for (;;) {
+    const int64_t off = atomic_fetch_add(w->next, w->morsel);
+    if (off >= w->end) break;
+    const int64_t stop = off + w->morsel < w->end ? off + w->morsel : w->end;
+#if Q1O_MUTANT == 14
+    if (stop - off < w->morsel) break; /* planted: the last partial morsel is dropped */
+#endif
+    for (int64_t i = off; i < stop; i++)
+      if (passes(w->c->shipdate[i], w->ccut)) aggregate_compressed(w->a, w->c, i);
+  }
+  return NULL;
+}
+
+void q1o_compressed_parallel(const q1o_compressed* c, int64_t begin, int64_t end, int32_t cutoff,
+                             int threads, int64_t morsel_rows, q1o_result* out) {
+  if (threads < 1) threads = 1;
+  if (morsel_rows < 1) morsel_rows = 1 << 20;
+  _Atomic int64_t next = begin;
+  cworker_t* ws = (cworker_t*)calloc((size_t)threads, sizeof(cworker_t));
+  pthread_t* ts = (pthread_t*)calloc((size_t)threads, sizeof(pthread_t));
+  for (int t = 0; t < threads; t++) {
+    ws[t].c = c; ws[t].end = end; ws[t].ccut = (int64_t)cutoff - 727563; ws[t].morsel = morsel_rows;
+    ws[t].next = &next;
+    pthread_create(&ts[t], NULL, compressed_worker, &ws[t]);
+  }
+  acc_t total[6];
+  memset(total, 0, sizeof(total));
+  for (int t = 0; t < threads; t++) {
+    pthread_join(ts[t], NULL);
+    for (int g = 0; g < 6; g++) {
+      total[g].sum_qty += ws[t].a[g].sum_qty;
+      total[g].sum_base_price += ws[t].a[g].sum_base_price;
+      total[g].sum_disc += ws[t].a[g].sum_disc;
+      total[g].sum_disc_price += ws[t].a[g].sum_disc_price;
+      total[g].sum_charge += ws[t].a[g].sum_charge;
+      total[g].count += ws[t].a[g].count;
+    }
+  }
+  free(ws);
+  free(ts);
+  acc_to_result(total, 0, out);
 }
 
 /* ---- filter pushdown producer (src/main.cu:362-389) and consumer ----------------------- */

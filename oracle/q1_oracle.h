@@ -79,6 +79,11 @@ void q1o_naive(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutof
 int q1o_hyper(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff,
               q1o_result* out);
 
+/* Morsel-parallel KernelHyPer<true> on `threads` threads: private overflow-checked tables
+ * per worker, merged with overflow-checked adds; returns 1 if any check fired. */
+int q1o_hyper_parallel(const q1o_lineitem* li, int64_t begin, int64_t end, int32_t cutoff, int threads,
+                       int64_t morsel_rows, q1o_result* out);
+
 /* Morsel-parallel KernelNaive on `threads` threads (pattern of Morsel::Query,
  * cpu/kernel_x100.hpp:456-500: workers claim fixed-size morsels with fetch_add and
  * merge their private tables).  Used as the timed CPU baseline. */
@@ -98,6 +103,12 @@ uint32_t q1o_get_line_status(const uint32_t* words, int64_t row);
  * quantity summed in whole units as the GPU kernels do. */
 void q1o_compressed_kernel(const q1o_compressed* c, int64_t begin, int64_t end, int32_t cutoff,
                            q1o_result* out);
+
+/* q1o_compressed_kernel morsel-parallel on `threads` threads (the morsel pattern of
+ * cpu/kernel_x100.hpp:456-500), for checking device runs over hundreds of millions of
+ * rows. */
+void q1o_compressed_parallel(const q1o_compressed* c, int64_t begin, int64_t end, int32_t cutoff,
+                             int threads, int64_t morsel_rows, q1o_result* out);
 
 /* precompute_filter_for_table_chunk (src/main.cu:362-389) with a parameter cutoff, over
  * rows [0, n); writes ceil(n/32) words. */

@@ -148,7 +148,28 @@ def check_hyper(o) -> bool:
         return False
     over = ref_columns([(CUTOFF, 1, 2**62, 0, 0, "A", "F")])  # (100 - 0) * 2^62 overflows int64
     _, flag = o.hyper(over, 0, 1, CUTOFF)
-    return flag == 1
+    if flag != 1:
+        return False
+    # morsel-parallel: the same sums, and an overflow inside any morsel (first, middle,
+    # last row) or only in the merge of the workers' tables raises the flag
+    big = random_rows(rng, 3001, tpch=True)
+    bcols = ref_columns(big)
+    want, _ = o.naive(bcols, 0, 3001, CUTOFF)
+    for threads, morsel in ((1, 100), (4, 77), (5, 1000)):
+        got, flag = o.hyper_parallel(bcols, 0, 3001, CUTOFF, threads, morsel)
+        if flag or got != want:
+            return False
+    for at in (0, 1500, 3000):
+        rows = list(big)
+        rows[at] = (CUTOFF, 1, 2**62, 0, 0, "A", "F")
+        if o.hyper_parallel(ref_columns(rows), 0, 3001, CUTOFF, 3, 100)[1] != 1:
+            return False
+    # one row fits, two rows' quantity sum passes 2^63 (whether one worker adds both rows or
+    # the merge adds two workers' tables)
+    near = ref_columns([(CUTOFF, 2**62, 1, 0, 0, "A", "F")] * 2)
+    _, f1 = o.hyper(near, 0, 1, CUTOFF)
+    _, f2 = o.hyper_parallel(near, 0, 2, CUTOFF, 2, 1)
+    return f1 == 0 and f2 == 1
 
 
 def _pack(codes, bits):
@@ -205,6 +226,25 @@ def check_compressed_kernel(o) -> bool:
                 g["sum_qty"] *= 100
             if not _same(sub, py_q1(rows[b:e], cut)):
                 return False
+    return True
+
+
+def check_compressed_parallel(o) -> bool:
+    # morsel-parallel compressed kernel vs the independent formulation: morsels and
+    # sub-ranges that start and end mid-word, more threads than morsels
+    rng = np.random.default_rng(29)
+    n = 3001
+    rows = random_rows(rng, n, tpch=True)
+    c = o.compress(ref_columns(rows), n)
+    for cut in (CUTOFF, 729906):
+        for b, e in ((0, n), (5, 2999), (17, 40)):
+            want = py_q1(rows[b:e], cut)
+            for threads, morsel in ((1, 100), (3, 77), (4, 1000), (7, 33)):
+                got = o.compressed_parallel(c, b, e, cut, threads, morsel)
+                for g in got:
+                    g["sum_qty"] *= 100
+                if not _same(got, want):
+                    return False
     return True
 
 

@@ -19,6 +19,7 @@ MUTANTS = {
     11: "brute_force",       # sum_disc of (1 - disc)
     12: "x100",              # X100 drops a morsel's last partial vector
     13: "avx512",            # AVX-512 final sum leaves out one lane
+    14: "compressed_parallel",  # compressed morsel-parallel drops a range's last partial morsel
 }
 
 
