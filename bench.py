@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--tuples-per-thread", type=int, default=0)
     ap.add_argument("--cpu-rows", type=int, default=60_000_000, help="CPU baseline sample (reference layout)")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-passes", type=int, default=6)
+    ap.add_argument("--cpu-passes", type=int, default=25,
+                    help="CPU baseline runs per kernel, the first (cold) one discarded (cpu/q1.cpp:268-299)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_compressed_600M_traffic.md"))
     ap.add_argument("--batch-requests", type=int, default=256, help="requests in the batch leg (0 skips it)")
@@ -47,9 +48,8 @@ def parse():
                     help="skip the pushdown / uncompressed / reference layouts leg")
     ap.add_argument("--no-host", dest="host", action="store_false",
                     help="skip the host-resident streaming and CPU/GPU split leg")
-    ap.add_argument("--serve-seconds", type=float, default=4.0,
-                    help="length of each offered rate in the serving leg (0 skips it)")
-    ap.add_argument("--serve-rates", default="2000,8000", help="offered request rates (req/s), comma-separated")
+    ap.add_argument("--serve-plan", default="2000:60,8000:5",
+                    help="serving leg: offered rate (req/s) and seconds, comma-separated pairs ('' skips it)")
     ap.add_argument("--copy-probe-bytes", type=int, default=0,
                     help="after the timed region, copy this many bytes twice with a plain device copy "
                          "kernel (scales rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -83,46 +83,86 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(q1, rows: int, threads: int, passes: int):
-    """The oracle's restatements of the reference CPU kernels on a bounded sample of the
-    same generator's rows (reference layout; ComprData for X100): KernelNaive
-    (cpu/kernel_naive.hpp) morsel-parallel on `threads` threads (the reported value) and on
-    one thread, KernelHyPer<true> (cpu/kernel_hyper.hpp) on one thread, and KernelX100
-    (cpu/kernel_x100.hpp, 1024-row vectors) on both.  Each is the mean of the hot passes
-    after one cold pass (cpu/q1.cpp:268-299).  Also returns the oracle's groups for these
-    rows, the checker of the device results on the same rows."""
+def cpu_baseline(q1, args, batch_spec):
+    """The oracle's restatements of the reference CPU kernels, timed on this host and used
+    as the checker of the device results (the only place the bench touches oracle/).
+
+    Protocol of cpu/q1.cpp:268-299: `args.cpu_passes` runs (default 25), the first (cold)
+    discarded, the mean of the rest.  Timed, on the reference layout (38 B/row) of the same
+    seeded generator the device reads:
+      * 6,000,000 rows (the reference's CPU case): KernelNaive (cpu/kernel_naive.hpp) and
+        KernelHyPer<true> (cpu/kernel_hyper.hpp) on one thread and morsel-parallel on
+        `args.cpu_threads` threads, KernelX100 (cpu/kernel_x100.hpp) on both;
+      * 60,000,000 rows: KernelNaive morsel-parallel;
+      * the whole 600,000,000-row scan workload: KernelNaive and KernelHyPer
+        morsel-parallel (the reported value is KernelNaive's rate here).
+    The library is the oracle built at the highest ISA level the host supports (the
+    reference builds with -march=native).  Returns the baseline and the oracle's groups for
+    the 600M scan, the 60M sample and every request of the batch."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle_lib import Oracle
+    from oracle_lib import Oracle, baseline_build
 
-    o = Oracle()
-    with q1.Context(0) as c:
-        t = c.generate(q1.REFERENCE, 0x51C00000, rows)
-        cols = t.download()
-        t.free()
-    comp = o.compact(cols, rows)
+    from tpchq01_gpu_amd import workloads
+    path, flags = baseline_build()
+    o = Oracle(path)
+    threads, passes = args.cpu_threads, max(2, args.cpu_passes)
+    cut = 729999
 
-    def timed(fn):
+    def timed(rows, fn):
         times = []
         for _ in range(passes):
             t0 = time.perf_counter()
             fn()
             times.append(time.perf_counter() - t0)
-        hot = times[1:] if len(times) > 1 else times
+        hot = times[1:]
         return rows / (sum(hot) / len(hot))
 
-    naive_mt = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, threads))
-    naive_1 = timed(lambda: o.naive_parallel(cols, 0, rows, 729999, 1))
-    hyper_1 = timed(lambda: o.hyper(cols, 0, rows, 729999))
-    x100_mt = timed(lambda: o.x100(comp, 0, rows, 729999, threads))
-    x100_1 = timed(lambda: o.x100(comp, 0, rows, 729999, 1))
-    want, _ = o.naive(cols, 0, rows, 729999)
-    return {"value": naive_mt, "unit": "rows/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} rows, reference layout (38 B/row), oracle KernelNaive restatement, "
-                      f"morsel-parallel on {threads} threads, mean of {passes - 1} hot passes",
-            "ms_per_pass": rows / naive_mt * 1e3,
-            "naive_1_thread_rows_s": naive_1, "hyper_1_thread_rows_s": hyper_1,
-            "x100_rows_s": x100_mt, "x100_1_thread_rows_s": x100_1,
-            "cpu_model": cpu_model(), "nproc": os.cpu_count()}, want
+    out = {"unit": "rows/s", "cores": threads, "kind": "port", "compiler_flags": flags,
+           "passes": passes, "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+    checks = {}
+    # 6M rows: the reference's CPU case, every kernel on one thread and on all threads
+    n = 6_000_000
+    cols = q1.generate_host(q1.REFERENCE, workloads.SAMPLE_SEED, n, 0, threads)
+    comp = o.compact(cols, n)
+    out["rows_6M"] = {
+        "naive_1_thread": timed(n, lambda: o.naive(cols, 0, n, cut)),
+        "hyper_1_thread": timed(n, lambda: o.hyper(cols, 0, n, cut)),
+        "x100_1_thread": timed(n, lambda: o.x100(comp, 0, n, cut, 1)),
+        "naive": timed(n, lambda: o.naive_parallel(cols, 0, n, cut, threads)),
+        "hyper": timed(n, lambda: o.hyper_parallel(cols, 0, n, cut, threads)),
+        "x100": timed(n, lambda: o.x100(comp, 0, n, cut, threads)),
+    }
+    del cols, comp
+    log(f"bench: cpu 6M {out['rows_6M']}")
+    # 60M rows: the every-placement sample
+    n = args.cpu_rows
+    cols = q1.generate_host(q1.REFERENCE, workloads.SAMPLE_SEED, n, 0, threads)
+    out["rows_60M"] = {"naive": timed(n, lambda: o.naive_parallel(cols, 0, n, cut, threads))}
+    checks["sample"] = o.naive_parallel(cols, 0, n, cut, threads)[0]
+    del cols
+    log(f"bench: cpu 60M {out['rows_60M']}")
+    # the whole scan workload
+    n = args.rows
+    cols = q1.generate_host(q1.REFERENCE, workloads.SCAN_SEED, n, 0, threads)
+    naive = timed(n, lambda: o.naive_parallel(cols, 0, n, cut, threads))
+    hyper = timed(n, lambda: o.hyper_parallel(cols, 0, n, cut, threads))
+    groups, flag = o.hyper_parallel(cols, 0, n, cut, threads)
+    checks["scan"] = o.naive_parallel(cols, 0, n, cut, threads)[0]
+    checks["scan_hyper_agrees"] = flag == 0 and groups == checks["scan"]
+    del cols
+    out.update({"value": naive, "ms_per_pass": n / naive * 1e3,
+                "sample": f"the whole scan workload: {n} rows in the reference layout (38 B/row), oracle "
+                          f"KernelNaive restatement morsel-parallel on {threads} threads, mean of "
+                          f"{passes - 1} passes after a cold one",
+                "rows_600M": {"naive": naive, "hyper": hyper}})
+    log(f"bench: cpu 600M {out['rows_600M']}")
+    # the batch's requests, each on its own rows (checker only)
+    if batch_spec:
+        checks["batch"] = []
+        for seed, rows, cutoff in batch_spec:
+            cols = q1.generate_host(q1.REFERENCE, seed, rows, 0, threads)
+            checks["batch"].append(o.naive_parallel(cols, 0, rows, cutoff, threads)[0])
+    return out, checks
 
 
 def hbm_probe(rows: int):
@@ -138,15 +178,17 @@ def hbm_probe(rows: int):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def batch_leg(q1, ctx, n: int):
-    """Batch leg: n requests, each on its own seeded column slice (log-uniform 1e4..2e7
-    rows, cutoff uniform in 1998-06-01..1998-11-30), in one q1_run_batch (persistent
-    kernel replayed from a hipGraph) vs the plain per-request q1_run one at a time."""
+def batch_leg(q1, ctx, spec):
+    """Batch leg: len(spec) requests, each on its own seeded column slice (log-uniform
+    1e4..2e7 rows, cutoff uniform in 1998-06-01..1998-11-30), in one q1_run_batch
+    (persistent kernel replayed from a hipGraph) vs the plain per-request q1_run one at a
+    time.  Returns the leg's figures and the batch's results (checked against the oracle
+    later, in the CPU baseline leg)."""
     import numpy as np
-    rng = np.random.default_rng(0x51C00003)
-    sizes = np.exp(rng.uniform(np.log(1e4), np.log(2e7), n)).astype(np.int64)
-    cutoffs = rng.integers(729906, 730089, n)
-    tables = [ctx.generate(q1.COMPRESSED, 0x51C00003 ^ r, int(sizes[r])) for r in range(n)]
+    n = len(spec)
+    sizes = np.array([s[1] for s in spec], dtype=np.int64)
+    cutoffs = np.array([s[2] for s in spec], dtype=np.int64)
+    tables = [ctx.generate(q1.COMPRESSED, spec[r][0], int(sizes[r])) for r in range(n)]
     reqs = (q1.Request * n)(*[q1.Request(tables[r].handle, 0, int(sizes[r]), int(cutoffs[r]), 0) for r in range(n)])
     outs = (q1.Result * n)()
     for _ in range(2):   # build the graph, then warm
@@ -165,13 +207,15 @@ def batch_leg(q1, ctx, n: int):
         singles.append(ctx.run_raw(tables[r], 0, int(sizes[r]), int(cutoffs[r]), q1.AUTO))
     one = time.perf_counter() - t0
     exact = all(q1.result_to_list(outs[r]) == q1.result_to_list(singles[r]) for r in range(n))
+    results = [as_dicts(q1.result_to_list(outs[r])) for r in range(n)]
     for t in tables:
         t.free()
     rows = int(sizes.sum())
     return {"requests": n, "rows": rows, "batched_ms": batched * 1e3, "batched_device_ms": batched_dev * 1e3,
             "batched_req_s": n / batched, "batched_rows_s": rows / batched,
+            "batched_device_gbps": rows * q1.BITS_PER_ROW[q1.COMPRESSED] / 8 / batched_dev / 1e9,
             "one_at_a_time_ms": one * 1e3, "one_at_a_time_req_s": n / one,
-            "speedup": one / batched, "exact_vs_single": exact}
+            "speedup": one / batched, "exact_vs_single": exact}, results
 
 
 def probe_ceiling(q1, probe, layout):
@@ -214,6 +258,20 @@ def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5, probe=
             out[name]["frac_of_read_ceiling"] = out[name]["gbps"] / ceiling
         t.free()
     out["exact_vs_compressed"] = exact
+    # the pushdown producer (q1_build_filter on the device: 2 B/row read, 1 bit/row written)
+    # against what each query saves by reading the bitmap instead of the ship dates
+    builds, comp_ms = [], []
+    for _ in range(reps):
+        pd = ctx.build_filter(comp_table, 729999)
+        builds.append(ctx.last_timing().kernel_ms)
+        pd.free()
+        ctx.run_raw(comp_table, 0, rows, 729999, q1.AUTO)
+        comp_ms.append(ctx.last_timing().kernel_ms)
+    build_ms, comp_ms = sum(builds) / len(builds), sum(comp_ms) / len(comp_ms)
+    saved = comp_ms - out["compressed_pushdown"]["kernel_ms"]
+    out["filter_producer"] = {"kernel_ms": build_ms, "gbps": rows * (2 + 1 / 8) / (build_ms * 1e-3) / 1e9,
+                              "compressed_kernel_ms": comp_ms, "saved_per_query_ms": saved,
+                              "queries_to_repay": build_ms / saved if saved > 0 else None}
     return out
 
 
@@ -310,20 +368,21 @@ def host_leg(q1, ctx, table, rows: int, seed: int, want, cpu_threads: int, reps:
         ctx.free_pinned_columns(cols)
 
 
-def serving_leg(q1, ctx, seconds: float, rates):
+def serving_leg(q1, ctx, plan):
     """Serving leg: Poisson arrivals over 8 HBM-resident compressed column sets
     (1M..600M rows), random row range and cutoff per request, through the
-    continuous-batching server vs one-at-a-time dispatch of the plain kernel, at each
-    offered rate in `rates`."""
+    continuous-batching server vs one-at-a-time dispatch of the plain kernel, for each
+    (offered rate, seconds) of `plan`."""
     import numpy as np
 
+    from tpchq01_gpu_amd import workloads
     from tpchq01_gpu_amd.serving import Server, make_request_stream, run_open_loop
-    sizes = np.geomspace(1e6, 6e8, 8).astype(np.int64)
-    tables = [ctx.generate(q1.COMPRESSED, 0x51C00004 + k, int(sizes[k])) for k in range(8)]
+    tables = [ctx.generate(q1.COMPRESSED, s, n) for s, n in workloads.serving_spec()]
     out = []
-    for rate in rates:
+    for rate, seconds in plan:
         stream = make_request_stream(tables, rate, seconds, seed=9)
-        with Server(ctx, max_batch=256, max_inflight=2, batch_window_us=200, memory_budget=64 << 20) as sv:
+        # admission budget: three quarters of device memory (the server's default)
+        with Server(ctx, max_batch=256, max_inflight=2, batch_window_us=0) as sv:
             done, wall = run_open_loop(sv, stream)
         lat = np.array([d[1] for d in done]) / 1e3
         bs = np.array([d[2] for d in done])
@@ -515,26 +574,35 @@ def main():
         log("bench: host-resident leg (streaming, CPU/GPU split)")
         out["host_resident"] = host_leg(q1, ctx, table, rows, 0x51C00002, one, args.cpu_threads)
         log(f"bench: host_resident {out['host_resident']}")
+    spec, batch_results = [], None
     if args.batch_requests:
+        from tpchq01_gpu_amd import workloads
+        spec = workloads.batch_spec(args.batch_requests)
         log(f"bench: batch leg, {args.batch_requests} requests")
-        out["batch"] = batch_leg(q1, ctx, args.batch_requests)
+        out["batch"], batch_results = batch_leg(q1, ctx, spec)
         log(f"bench: batch {out['batch']}")
-    if args.serve_seconds > 0:
-        rates = [float(x) for x in args.serve_rates.split(",") if x]
-        log(f"bench: serving leg, {rates} req/s for {args.serve_seconds}s each")
-        out["serving"] = serving_leg(q1, ctx, args.serve_seconds, rates)
-    if not args.no_cpu:
-        log(f"bench: CPU baseline on {args.cpu_rows} rows x {args.cpu_passes} passes, {args.cpu_threads} threads")
-        out["cpu_baseline"], want = cpu_baseline(q1, args.cpu_rows, args.cpu_threads, args.cpu_passes)
-        if "uncompressed_60M" in out and out["uncompressed_60M"]["rows"] == args.cpu_rows:
-            out["uncompressed_60M"]["exact_vs_oracle"] = all(
-                as_dicts(r) == want for r in out["uncompressed_60M"].pop("_results"))
-    if "uncompressed_60M" in out:
-        out["uncompressed_60M"].pop("_results", None)
-    else:
-        out["cpu_baseline"] = None
+    plan = [(float(r), float(s)) for r, s in (x.split(":") for x in args.serve_plan.split(",") if x)]
+    if plan:
+        log(f"bench: serving leg, {plan} (req/s : seconds)")
+        out["serving"] = serving_leg(q1, ctx, plan)
     table.free()
     ctx.close()
+    out["cpu_baseline"] = None
+    if not args.no_cpu:
+        log(f"bench: CPU baseline ({args.cpu_passes} passes, {args.cpu_threads} threads) and oracle checks")
+        out["cpu_baseline"], checks = cpu_baseline(q1, args, spec)
+        # every device result above against the oracle's groups for the same rows
+        out["exact_vs_oracle"] = as_dicts(one) == checks["scan"] and checks["scan_hyper_agrees"]
+        if "uncompressed_60M" in out and out["uncompressed_60M"]["rows"] == args.cpu_rows:
+            out["uncompressed_60M"]["exact_vs_oracle"] = all(
+                as_dicts(r) == checks["sample"] for r in out["uncompressed_60M"]["_results"])
+        if batch_results is not None:
+            out["batch"]["exact_vs_oracle"] = batch_results == checks["batch"]
+        log(f"bench: exact vs oracle: scan {out['exact_vs_oracle']}, "
+            f"batch {out.get('batch', {}).get('exact_vs_oracle')}, "
+            f"60M {out.get('uncompressed_60M', {}).get('exact_vs_oracle')}")
+    if "uncompressed_60M" in out:
+        out["uncompressed_60M"].pop("_results", None)
     print(json.dumps(out), flush=True)
 
 

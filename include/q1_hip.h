@@ -176,9 +176,14 @@ void* q1_stream(q1_ctx* ctx); /* (new) the context's hipStream_t */
 /* Copies host columns to device memory (pinned staging, timed in q1_last_timing.h2d_ms). */
 q1_status q1_table_upload(q1_ctx* ctx, q1_layout layout, const q1_columns* host, uint64_t rows,
                           q1_table** out);
-/* Wraps caller-owned device columns (16-byte aligned); nothing is copied or freed. */
+/* Wraps caller-owned device columns (16-byte aligned); nothing is copied or freed.  The
+ * columns' value domain (which selects the narrow kernels) is measured once, here: a
+ * caller that later writes the columns calls q1_table_refresh_domain before the next run. */
 q1_status q1_table_wrap(q1_ctx* ctx, q1_layout layout, const q1_columns* device, uint64_t rows,
                         q1_table** out);
+/* (new) Re-measures a table's value domain on the device (after its caller-owned columns
+ * were written). */
+q1_status q1_table_refresh_domain(q1_ctx* ctx, q1_table* t);
 /* (new) Generates a synthetic table on the device with the library's seeded generator. */
 q1_status q1_table_generate(q1_ctx* ctx, q1_layout layout, uint64_t seed, uint64_t rows,
                             q1_table** out);
@@ -231,7 +236,7 @@ typedef struct {
  * threads_per_block must be 0 or 256; tuples_per_thread (rows per lane of each block of a
  * request; 0 = the tuned value) a multiple of 16, at most 256 for the shared placement and
  * 16384 for the register one; num_blocks 0 = as many blocks as are resident at once
- * (at most 2^24). */
+ * (below 2^24). */
 q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t n, const q1_launch* launch,
                        q1_result* outs);
 
@@ -252,6 +257,12 @@ q1_status q1_compress_columns(const q1_columns* reference, uint64_t rows,
  * ship_date column (src/main.cu:362-389 semantics, any cutoff). */
 q1_status q1_cpu_build_filter(const uint16_t* compressed_ship_date, uint64_t rows,
                               int32_t cutoff_day, uint32_t* filter_out, int threads);
+/* (new) Whether host columns of `layout` lie inside the TPC-H value domain (discount and
+ * tax 0..15, quantity 0..63 units, extended_price below 2^24), i.e. whether a caller may
+ * pass Q1_LAUNCH_ASSUME_TPCH_DOMAIN for them; *in_domain = 1 or 0.  Tables created
+ * through this library measure their own domain on the device. */
+q1_status q1_host_columns_in_tpch_domain(q1_layout layout, const q1_columns* host, uint64_t rows, int threads,
+                                         int* in_domain);
 /* MonetDB day number of a calendar date (cpu/monetdb.cpp:61-77 numbering). */
 int32_t q1_day_number(int32_t year, int32_t month, int32_t day);
 
@@ -285,8 +296,13 @@ typedef struct {
   uint32_t max_batch;          /* requests per batch launch (<= 1024) */
   uint32_t max_inflight;       /* batches on the device at once (streams) */
   uint32_t queue_capacity;     /* pending requests before Q1_EBUSY */
-  uint32_t batch_window_us;    /* how long the scheduler waits to fill a batch */
-  uint64_t memory_budget;      /* admission: device bytes for in-flight request state */
+  uint32_t batch_window_us;    /* how long the scheduler may hold a batch back to fill it while
+                                  another batch is on the device (0: launch at once) */
+  uint64_t memory_budget;      /* admission: device bytes the in-flight work may pin -- the
+                                  columns of every table with requests in flight (once per
+                                  table) plus 480 B of descriptor and result per request;
+                                  0 = three quarters of the device's memory.  A request whose
+                                  table alone does not fit gets Q1_ENOMEM at submit. */
 } q1_server_config;
 
 typedef struct {

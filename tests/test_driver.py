@@ -115,6 +115,40 @@ def test_driver_tbl_ingest_and_bin_cache(oracle, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("flags", [["--apply-compression"], ["--apply-compression", "--device-resident"],
+                                   ["--apply-compression", "--use-filter-pushdown", "--filter-once"], []])
+def test_driver_tbl_outside_the_domain(oracle, tmp_path, flags):
+    """A .tbl file with quantities of 99 units and prices of 2^24 cents or more (outside the
+    TPC-H domain the narrow kernels assume): the driver checks the ingested columns, runs
+    the 64-bit kernels, and prints the oracle's result."""
+    rows = 20_011
+    ref = q1.generate_host(q1.REFERENCE, 0x51C00022, rows)
+    ref["quantity"][::7] = 9900
+    ref["extended_price"][3::11] = 1 << 25
+    d = tmp_path / "data" / "1.000000"
+    d.mkdir(parents=True)
+    write_tbl(d / "lineitem.tbl", ref, rows)
+    r = run(["--data-dir=%s" % (tmp_path / "data"), "--runs=2", "--print-results"] + flags, cwd=tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "Values outside the TPC-H domain" in r.stdout
+    assert parse_printed(r.stdout) == expected(oracle, ref, rows)
+
+
+@pytest.mark.gpu
+def test_driver_filter_once(oracle, tmp_path):
+    """--filter-once builds the pushdown bitmap before the timed runs (host and device)."""
+    rows = 1_000_003
+    ref = q1.generate_host(q1.REFERENCE, 0x51C00023, rows)
+    for extra in ([], ["--device-resident"], ["--device-resident", "--use-coprocessing"]):
+        r = run(["--rows=%d" % rows, "--runs=2", "--print-results", "--seed=0x51C00023", "--apply-compression",
+                 "--use-filter-pushdown", "--filter-once"] + extra, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        assert "Filter bitmap built once" in r.stdout
+        assert parse_printed(r.stdout) == expected(oracle, ref, rows), extra
+
+
+@pytest.mark.gpu
 def test_driver_device_info():
     r = run(["--device"])
     assert r.returncode == 1 and "gfx950" in r.stdout

@@ -268,7 +268,7 @@ def test_batch_rows_per_lane_and_empty_requests(ctx):
     others and last) own no block, so the search for a block's request must pass over
     them.  Request sizes straddle the block size; every grid and rows per lane gives the
     results of the requests run alone.  Values that are not a multiple of 16, or above
-    16384, are refused, and so are grids above 2^24 blocks."""
+    16384, are refused, and so are grids of 2^24 blocks or more (2^32 work-items)."""
     t = ctx.generate(q1.COMPRESSED, SEED + 33, 300_000)
     spans = [(0, 0), (5, 1), (0, 4095), (7, 4096), (9, 4097), (100, 0), (13, 65_549), (0, 300_000),
              (299_999, 1), (1234, 0), (50_001, 123_457), (3, 0)]
@@ -284,7 +284,7 @@ def test_batch_rows_per_lane_and_empty_requests(ctx):
                                           C.byref(q1.make_launch(0, tpt, blocks)), outs), "q1_run_batch")
             for i in range(n):
                 assert q1.result_to_list(outs[i]) == want[i], (tpt, blocks, i)
-    for bad_tpt, bad_blocks in ((24, 0), (16400, 0), (0, (1 << 24) + 1)):
+    for bad_tpt, bad_blocks in ((24, 0), (16400, 0), (0, 1 << 24), (0, (1 << 24) + 1)):
         outs = (q1.Result * n)()
         st = q1.lib.q1_run_batch(ctx.handle, (q1.Request * n)(*reqs), n,
                                  C.byref(q1.make_launch(0, bad_tpt, bad_blocks)), outs)

@@ -200,6 +200,18 @@ def test_serving_mix_2000_req_s(ctx, oracle):
             st, groups = got[i]
             assert st == q1.OK, i
             assert groups == ctx.run(t, b, n, c), i
+        # admission by device memory at this scale: a budget that holds every table but the
+        # 600M-row one refuses requests on that table at submit and serves the others
+        sizes = [sum(q1.column_bytes(t.layout, t.rows)) for t in tables]
+        with Server(ctx, max_batch=256, max_inflight=2, memory_budget=sum(sizes[:-1]) + (1 << 20)) as sv:
+            for k, t in enumerate(tables):
+                st, tk = sv.submit(t, 0, t.rows, 729999)
+                if k == len(tables) - 1:
+                    assert st == q1.ENOMEM and sizes[k] > sum(sizes[:-1])
+                else:
+                    assert st == q1.OK
+                    st, groups, _ = sv.wait(tk)
+                    assert st == q1.OK and groups == ctx.run(t, 0, t.rows, 729999), k
         rng = np.random.default_rng(0x51C0)
         sample = sorted(rng.choice(len(stream), 200, replace=False))
         index = {t.handle: k for k, t in enumerate(tables)}

@@ -156,6 +156,16 @@ def test_device_filter_matches_oracle(ctx, oracle):
         pd = ctx.build_filter(t, cutoff)
         got = pd.download()["filter"]
         assert np.array_equal(got, oracle.precompute_filter(comp["ship_date"], rows, cutoff))
+    # sizes around the producer's 8-row lanes and 32-row words
+    for n in (1, 7, 8, 9, 31, 32, 33, 255, 257, 2049):
+        c = {k: v[: (n + 15) // 16 if k == "return_flag" else (n + 31) // 32 if k == "line_status" else n].copy()
+             for k, v in comp.items()}
+        tn = ctx.upload(q1.COMPRESSED, c, n)
+        for cutoff in (729999, 729906):
+            pd = ctx.build_filter(tn, cutoff)
+            assert np.array_equal(pd.download()["filter"], oracle.precompute_filter(c["ship_date"], n, cutoff)), n
+            pd.free()
+        tn.free()
 
 
 def test_uploaded_pushdown_columns(ctx, oracle):

@@ -70,9 +70,16 @@ def test_concurrent_submitters(ctx, tables):
         assert st == q1.OK and groups == ctx.run(t, b, n, c)
 
 
+def table_bytes(t):
+    return sum(q1.column_bytes(t.layout, t.rows))
+
+
 def test_admission_by_memory_budget(ctx, tables):
+    """In-flight work pins its tables' columns (once per table) and 480 B of descriptor and
+    result per request: a budget of one table plus four requests' state keeps at most four
+    requests on the device at once."""
     per_request = 96 + 384   # descriptor + result table bytes on the device
-    with Server(ctx, max_batch=256, max_inflight=2, memory_budget=4 * per_request) as sv:
+    with Server(ctx, max_batch=256, max_inflight=2, memory_budget=table_bytes(tables[1]) + 4 * per_request) as sv:
         tickets = []
         for i in range(60):
             st, tk = sv.submit(tables[1], i * 1000, 500_000, 729999)

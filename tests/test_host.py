@@ -233,3 +233,35 @@ def test_cpu_coprocessor_compressed_full_range_random():
         w["sum_charge"] += dp * (100 + int(tax[i]))
         w["count"] += 1
     assert as_dicts(got) == want
+
+
+@pytest.mark.parametrize("layout", [q1.REFERENCE, q1.UNCOMPRESSED, q1.COMPRESSED])
+def test_host_domain_check(layout):
+    """q1_host_columns_in_tpch_domain: generated rows lie inside the TPC-H value domain; one
+    row beyond a bound of any of the four measured columns (discount or tax above 15,
+    quantity above 63 units, price at 2^24 or more, a negative discount) puts the columns
+    outside it."""
+    import ctypes as C
+
+    def check(cols, n):
+        flag = C.c_int(-1)
+        st = q1.lib.q1_host_columns_in_tpch_domain(layout, C.byref(q1.columns_struct(cols)), n, 3, C.byref(flag))
+        assert st == q1.OK
+        return flag.value
+
+    n = 10_007
+    base = q1.generate_host(layout, SEED, n)
+    assert check(base, n) == 1
+    assert check(base, 0) == 1
+    qscale = 1 if layout == q1.COMPRESSED else 100
+    for col, value in (("discount", 16), ("tax", 16), ("quantity", 64 * qscale), ("extended_price", 1 << 24)):
+        for row in (0, n // 2, n - 1):
+            cols = {k: v.copy() for k, v in base.items()}
+            cols[col][row] = value
+            assert check(cols, n) == 0, (col, row)
+            # the row past the range is not looked at
+            assert check(cols, row) == 1, (col, row)
+    if layout != q1.COMPRESSED:
+        cols = {k: v.copy() for k, v in base.items()}
+        cols["discount"][7] = -1
+        assert check(cols, n) == 0

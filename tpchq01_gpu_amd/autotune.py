@@ -2,21 +2,33 @@
 scripts/try_grid_params.py and the hand-entered optimum table of
 scripts/benchmark_with_optimal_params.py:39-55).
 
-For every (layout, placement) it sweeps threads_per_block x tuples_per_thread, times the
-kernel alone with HIP events (q1_last_timing().kernel_ms, the minimum over repetitions),
-skips shapes that leave fewer than 2 blocks per compute unit (as try_grid_params.py:78-80
-does), rotates between table copies when the table is smaller than the 256 MB Infinity
-Cache so the timing reflects HBM, and writes
+Scan shapes: for every (layout, placement) it sweeps threads_per_block x tuples_per_thread,
+times the kernel alone with HIP events (q1_last_timing().kernel_ms, the minimum over
+repetitions), skips shapes that leave fewer than 2 blocks per compute unit (as
+try_grid_params.py:78-80 does), and rotates between table copies when the table is
+smaller than the 256 MB Infinity Cache so the timing reflects HBM.
 
-  tuning/gfx950.txt   "launch <layout> <placement> <tpb> <tpt>" and "best <layout> <placement>"
-                      lines, read by q1_init (or Q1_TUNING_FILE)
-  tuning/gfx950.json  every measurement
+Batch shapes: for each batch placement (the register table and the LDS table) it sweeps
+the batched kernel's grid (blocks per compute unit) x rows per lane on the 256-request
+batch workload (uneven log-uniform requests, their own cutoffs), timing q1_run_batch's
+kernel.
 
-Usage: python -m tpchq01_gpu_amd.autotune [--rows N] [--layouts compressed,...]
+It writes
+  tuning/<arch>.txt    "launch <layout> <placement> <tpb> <tpt>", "best <layout> <placement>"
+                       and "batch <layout> <placement> <blocks per CU> <rows per lane>"
+                       lines, read by q1_init (or Q1_TUNING_FILE)
+  tuning/<arch>.json   every measurement
+and `--report` prints the optimum per layout and placement as a table
+(tuning/<arch>_report.md), the place of the reference's per-configuration optimum list and
+its plots (scripts/benchmark_with_optimal_params.py:39-55, scripts/plots.R).
+
+Usage: python -m tpchq01_gpu_amd.autotune [--rows N] [--layouts compressed,...] [--batch]
+       [--merge] [--report]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -26,11 +38,16 @@ from . import PACKAGE_DIR, q1
 
 THREADS = [64, 128, 256]
 TUPLES = [128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+BATCH_BLOCKS_PER_CU = [1, 2, 3, 4, 6, 8]
+BATCH_LANE_ROWS = {q1.IN_REGISTERS: [256, 512, 1024, 2048], q1.PER_THREAD_SHARED_MEM: [64, 128, 192, 256]}
 MALL_BYTES = 256 << 20
+HBM_PEAK_GBS = 8000.0
+LAYOUT_KEYS = {"reference": q1.REFERENCE, "uncompressed": q1.UNCOMPRESSED, "compressed": q1.COMPRESSED,
+               "compressed_pushdown": q1.COMPRESSED_PUSHDOWN}
 
 
 def sweep(ctx: q1.Context, layout: int, rows: int, placements, reps: int, seed: int = 0x51C00010,
-          log=print):
+          log=print, threads=THREADS, tuples=TUPLES):
     algo = rows * q1.BITS_PER_ROW[layout] // 8
     copies = 1 if algo >= 4 * MALL_BYTES else max(2, (4 * MALL_BYTES) // max(algo, 1) + 1)
     copies = min(copies, 8)
@@ -44,15 +61,14 @@ def sweep(ctx: q1.Context, layout: int, rows: int, placements, reps: int, seed: 
     results = []
     try:
         for p in placements:
-            for tpb in THREADS:
-                for tpt in TUPLES:
+            for tpb in threads:
+                for tpt in tuples:
                     blocks = -(-rows // (tpb * tpt))
                     if blocks < 2 * cus:
                         continue
                     launch = q1.make_launch(tpb, tpt)
                     best = float("inf")
-                    # the memory-bound-by-atomics placements are 20-100x slower: one run each
-                    for r in range(1 if p in (q1.LOCAL_MEM, q1.GLOBAL) else reps):
+                    for r in range(reps):
                         ctx.run_raw(tables[r % copies], 0, rows, 729999, p, launch)
                         best = min(best, ctx.last_timing().kernel_ms)
                     gbs = algo / (best * 1e-3) / 1e9
@@ -60,6 +76,40 @@ def sweep(ctx: q1.Context, layout: int, rows: int, placements, reps: int, seed: 
                                         kernel_ms=best, gbps=gbs, rows=rows))
                     log(f"{q1.LAYOUT_NAMES[layout]:>20} {q1.PLACEMENT_NAMES[p]:>24} tpb={tpb:4d} tpt={tpt:5d} "
                         f"blocks={blocks:6d} {best:8.3f} ms {gbs:7.1f} GB/s")
+    finally:
+        for t in tables:
+            t.free()
+    return results
+
+
+def sweep_batch(ctx: q1.Context, layout: int, spec, placements, reps: int, log=print,
+                blocks_per_cu=BATCH_BLOCKS_PER_CU, lane_rows=BATCH_LANE_ROWS):
+    """The batched kernel's grid x rows per lane on the requests of `spec` ((seed, rows,
+    cutoff) per request, each on its own table), per batch placement."""
+    tables = [ctx.generate(layout, s, n) for s, n, _ in spec]
+    n = len(spec)
+    reqs = (q1.Request * n)(*[q1.Request(tables[r].handle, 0, spec[r][1], spec[r][2], 0) for r in range(n)])
+    outs = (q1.Result * n)()
+    rows = sum(s[1] for s in spec)
+    algo = rows * q1.BITS_PER_ROW[layout] // 8
+    cus = ctx.device_info().compute_units
+    results = []
+    try:
+        for p in placements:
+            ctx.set_best_placement(layout, p)    # the batch runs the layout's best placement
+            for per_cu in blocks_per_cu:
+                for lr in lane_rows[p]:
+                    launch = q1.make_launch(0, lr, per_cu * cus)
+                    best = float("inf")
+                    for r in range(reps + 1):    # the first call captures the graph
+                        ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, C.byref(launch), outs), "q1_run_batch")
+                        if r:
+                            best = min(best, ctx.last_timing().kernel_ms)
+                    gbs = algo / (best * 1e-3) / 1e9
+                    results.append(dict(layout=layout, placement=p, blocks_per_cu=per_cu, tpt=lr,
+                                        blocks=per_cu * cus, requests=n, rows=rows, kernel_ms=best, gbps=gbs))
+                    log(f"batch {q1.LAYOUT_NAMES[layout]:>20} {q1.PLACEMENT_NAMES[p]:>24} blocks/CU={per_cu} "
+                        f"tpt={lr:5d} {best:8.3f} ms {gbs:7.1f} GB/s")
     finally:
         for t in tables:
             t.free()
@@ -79,44 +129,111 @@ def best_table(results):
     return best, per_layout
 
 
-def write_tables(results, out_dir: str, arch: str = "gfx950"):
+def merge_results(old, new):
+    """Measurements of `old` whose (layout, placement) `new` does not sweep again, plus `new`."""
+    swept = {(r["layout"], r["placement"]) for r in new}
+    return [r for r in old if (r["layout"], r["placement"]) not in swept] + list(new)
+
+
+def write_tables(results, out_dir: str, device: str, rows: int, batch_results=(), arch: str = "gfx950"):
+    """Writes tuning/<arch>.txt (read by q1_init) and tuning/<arch>.json."""
     os.makedirs(out_dir, exist_ok=True)
     best, per_layout = best_table(results)
+    bbest, _ = best_table(batch_results)
     with open(os.path.join(out_dir, f"{arch}.txt"), "w") as f:
-        f.write("# written by tpchq01_gpu_amd.autotune: launch <layout> <placement> <tpb> <tpt> / best <layout> <placement>\n")
+        f.write(f"# written by tpchq01_gpu_amd.autotune on {device}, every layout swept at {rows} "
+                f"device-resident rows:\n")
+        f.write("#   launch <layout> <placement> <threads_per_block> <tuples_per_thread>   /   best <layout> <placement>\n")
+        f.write("#   batch <layout> <placement> <blocks_per_cu> <tuples_per_thread>   (the batched kernel, 256 threads)\n")
+        f.write("# (layouts 0 reference, 1 uncompressed, 2 compressed, 3 compressed + pushdown bitmap)\n")
         for (l, p), r in sorted(best.items()):
             f.write(f"launch {l} {p} {r['tpb']} {r['tpt']}\n")
         for l, r in sorted(per_layout.items()):
             f.write(f"best {l} {r['placement']}\n")
+        for (l, p), r in sorted(bbest.items()):
+            f.write(f"batch {l} {p} {r['blocks_per_cu']} {r['tpt']}\n")
     with open(os.path.join(out_dir, f"{arch}.json"), "w") as f:
-        json.dump({"measurements": results,
+        json.dump({"device": device, "rows": rows, "measurements": list(results),
+                   "batch_measurements": list(batch_results),
                    "best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
-                            for (l, p), r in sorted(best.items())]}, f, indent=1)
+                            for (l, p), r in sorted(best.items())],
+                   "batch_best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
+                                  for (l, p), r in sorted(bbest.items())]}, f, indent=1)
     return best, per_layout
+
+
+def report(table: dict) -> str:
+    """Markdown: the optimum launch per layout and placement, with its kernel time, rate and
+    fraction of HBM peak, a bar per placement, and the batched kernel's optimum."""
+    best, per_layout = best_table(table["measurements"])
+    lines = [f"# Tuned launch shapes on {table.get('device', '?')} ({table.get('rows', '?')} rows per layout)", "",
+             "| layout | placement | threads/block | tuples/thread | kernel ms | GB/s | % of 8 TB/s | |",
+             "|---|---|---|---|---|---|---|---|"]
+    for (l, p), r in sorted(best.items()):
+        frac = r["gbps"] / HBM_PEAK_GBS
+        mark = " **best**" if per_layout[l] is r else ""
+        lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]}{mark} | {r['tpb']} | {r['tpt']} | "
+                     f"{r['kernel_ms']:.3f} | {r['gbps']:.0f} | {100 * frac:.1f} | `{'#' * round(40 * frac)}` |")
+    bbest, _ = best_table(table.get("batch_measurements", []))
+    if bbest:
+        lines += ["", "Batched kernel (256 threads per block):", "",
+                  "| layout | placement | blocks/CU | tuples/thread | requests | rows | kernel ms | GB/s | % of 8 TB/s |",
+                  "|---|---|---|---|---|---|---|---|---|"]
+        for (l, p), r in sorted(bbest.items()):
+            lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]} | {r['blocks_per_cu']} | {r['tpt']} | "
+                         f"{r['requests']} | {r['rows']} | {r['kernel_ms']:.3f} | {r['gbps']:.0f} | "
+                         f"{100 * r['gbps'] / HBM_PEAK_GBS:.1f} |")
+    return "\n".join(lines) + "\n"
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rows", type=int, default=600_000_000)
-    ap.add_argument("--layouts", default="compressed")
+    ap.add_argument("--layouts", default="compressed", help="layouts whose scan shapes are swept ('' = none)")
     ap.add_argument("--placements", default="in_registers,in_registers_per_thread,local_mem,shared_mem_per_thread,global")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--batch", action="store_true", help="also sweep the batched kernel's shapes (compressed)")
+    ap.add_argument("--batch-requests", type=int, default=256)
+    ap.add_argument("--merge", action="store_true",
+                    help="keep the existing table's measurements of what this run does not sweep")
+    ap.add_argument("--report", action="store_true", help="print (and write) the report of the existing table")
     ap.add_argument("--out", default=os.path.join(PACKAGE_DIR, "tuning"))
     args = ap.parse_args(argv)
-    names = {"reference": q1.REFERENCE, "uncompressed": q1.UNCOMPRESSED, "compressed": q1.COMPRESSED,
-             "compressed_pushdown": q1.COMPRESSED_PUSHDOWN}
-    layouts = [names[x] for x in args.layouts.split(",")]
+    json_path = os.path.join(args.out, "gfx950.json")
+    if args.report:
+        with open(json_path) as f:
+            text = report(json.load(f))
+        with open(os.path.join(args.out, "gfx950_report.md"), "w") as f:
+            f.write(text)
+        print(text, end="")
+        return 0
+    layouts = [LAYOUT_KEYS[x] for x in args.layouts.split(",") if x]
     placements = [q1.PLACEMENTS[x] for x in args.placements.split(",")]
+    old = {"measurements": [], "batch_measurements": []}
+    if args.merge and os.path.exists(json_path):
+        with open(json_path) as f:
+            old = json.load(f)
     t0 = time.time()
+    out = lambda s: print(s, flush=True)
     with q1.Context(0) as ctx:
+        info = ctx.device_info()
+        device = f"MI355X ({info.arch.decode().split(':')[0]})"
         results = []
         for l in layouts:
-            results += sweep(ctx, l, args.rows, placements, args.reps, log=lambda s: print(s, flush=True))
-    best, per_layout = write_tables(results, args.out)
+            results += sweep(ctx, l, args.rows, placements, args.reps, log=out)
+        batch_results = []
+        if args.batch:
+            from . import workloads
+            batch_results = sweep_batch(ctx, q1.COMPRESSED, workloads.batch_spec(args.batch_requests),
+                                        [q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM], args.reps, log=out)
+    results = merge_results(old.get("measurements", []), results)
+    batch_results = merge_results(old.get("batch_measurements", []), batch_results)
+    best, per_layout = write_tables(results, args.out, device, args.rows, batch_results)
     for l, r in per_layout.items():
         print(f"best {q1.LAYOUT_NAMES[l]}: {q1.PLACEMENT_NAMES[r['placement']]} tpb={r['tpb']} tpt={r['tpt']} "
               f"{r['kernel_ms']:.3f} ms {r['gbps']:.1f} GB/s", flush=True)
-    print(f"autotune: {len(results)} shapes in {time.time() - t0:.1f}s", flush=True)
+    print(f"autotune: {len(results)} scan shapes, {len(batch_results)} batch shapes in {time.time() - t0:.1f}s",
+          flush=True)
     return 0
 
 

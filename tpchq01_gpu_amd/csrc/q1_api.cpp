@@ -169,6 +169,15 @@ static void load_tuning_file(q1_ctx* ctx) {
     } else if (kind == "best") {
       ls >> l >> p;
       if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL) ctx->best[l] = (q1_placement)p;
+    } else if (kind == "batch") {
+      // batch <layout> <placement> <blocks per compute unit> <rows per lane>
+      uint32_t per_cu = 0, lane_rows = 0;
+      ls >> l >> p >> per_cu >> lane_rows;
+      const bool shape_ok = per_cu >= 1 && per_cu <= 64 && lane_rows && lane_rows % 16 == 0 && lane_rows <= 16384;
+      if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL && shape_ok) {
+        ctx->batch_tuning[l][p].num_blocks = per_cu;
+        ctx->batch_tuning[l][p].tuples_per_thread = lane_rows;
+      }
     }
   }
 }
@@ -423,6 +432,20 @@ q1_status q1_table_wrap(q1_ctx* ctx, q1_layout layout, const q1_columns* device,
   t->fast = tmp.fast;
   ctx->tables.insert(t);
   *out = t;
+  return Q1_OK;
+  GUARD_END(ctx)
+}
+
+q1_status q1_table_refresh_domain(q1_ctx* ctx, q1_table* t) {
+  if (!ctx || !t || t->ctx != ctx) return fail(ctx, Q1_EINVAL, "bad argument");
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> g(ctx->lock);
+  q1_table tmp = *t;
+  if (t->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN) tmp.layout = Q1_LAYOUT_COMPRESSED;
+  const q1_status s = measure_domain(ctx, &tmp);
+  if (s != Q1_OK) return s;
+  t->domain = tmp.domain;
+  t->fast = tmp.fast;
   return Q1_OK;
   GUARD_END(ctx)
 }

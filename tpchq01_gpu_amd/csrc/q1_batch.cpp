@@ -39,16 +39,24 @@ void batch_destroy(q1_batch_state* b) {
 
 static uint64_t buf_bytes(uint32_t cap) { return sizeof(BatchHeader) + (uint64_t)cap * sizeof(BatchDesc); }
 
+// Rows per lane: the autotuner's batch value for the layout and placement, else the
+// scan's tuned value, within the batched kernel's limit.
 uint32_t batch_lane_rows(q1_ctx* ctx, q1_layout layout, q1_placement p) {
   p = batch_placement(p);
-  const uint32_t tuned = ctx->tuning[layout][p].tuples_per_thread;
   const uint32_t most = batch_max_lane_rows(p);
+  const uint32_t batch_tuned = ctx->batch_tuning[layout][p].tuples_per_thread;
+  if (batch_tuned >= 16 && batch_tuned <= most) return batch_tuned / 16 * 16;
+  const uint32_t tuned = ctx->tuning[layout][p].tuples_per_thread;
   return tuned >= 16 && tuned <= most ? tuned / 16 * 16 : std::min<uint32_t>(256, most);
 }
 
+// Grid: the autotuner's blocks per compute unit for the layout and placement, else as many
+// blocks as are resident at once.
 uint32_t batch_default_blocks(q1_ctx* ctx, q1_layout layout, q1_placement p) {
+  p = batch_placement(p);
   const uint32_t cus = (uint32_t)(ctx->compute_units > 0 ? ctx->compute_units : 256);
-  return cus * batch_blocks_per_cu(layout, p);
+  const uint32_t per_cu = ctx->batch_tuning[layout][p].num_blocks;
+  return cus * (per_cu ? per_cu : batch_blocks_per_cu(layout, p));
 }
 
 uint64_t batch_describe(BatchDesc* d, const q1_request& r, uint64_t first, uint32_t lane_rows) {
@@ -162,8 +170,9 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     }
     const uint32_t blocks =
         launch && launch->num_blocks ? launch->num_blocks : batch_default_blocks(ctx, layout, p);
-    // The kernel's claims count in 32 bits from gridDim.x up to the number of blocks of rows.
-    if (blocks > (1u << 24)) return fail(ctx, Q1_EINVAL, "batched num_blocks must be at most 2^24");
+    // The grid's work-items (blocks x 256) must stay below 2^32, and the kernel's claims
+    // count in 32 bits from gridDim.x up to the number of blocks of rows.
+    if (blocks >= (1u << 24)) return fail(ctx, Q1_EINVAL, "batched num_blocks must be below 2^24");
     uint32_t cap = 16;
     while (cap < n) cap <<= 1;
     q1_status s = build_state(ctx, layout, p, cap, threads, blocks);
@@ -195,6 +204,7 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     ctx->last.bytes_read = algorithmic_bytes(layout, rows);
     ctx->last.num_blocks = blocks;
     ctx->last.threads_per_block = threads;
+    ctx->last.tuples_per_thread = lane_rows;
     ctx->last.placement = (uint32_t)p;
     return Q1_OK;
   } catch (const std::exception& e) {

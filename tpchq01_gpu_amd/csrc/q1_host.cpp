@@ -3,12 +3,14 @@
 // CPU coprocessor (morsel-parallel aggregation in the manner of cpu/kernel_x100.hpp:456-500:
 // workers claim fixed-size morsels with an atomic counter and merge private tables).
 #include <atomic>
+#include <cstdint>
 #include <cstring>
 #include <thread>
 #include <vector>
 
 #include "../../include/q1_hip.h"
 #include "q1_common.h"
+#include "q1_kernels.h"
 
 namespace {
 
@@ -259,6 +261,58 @@ q1_status q1_cpu_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutof
   } catch (...) {
     return Q1_ENOMEM;
   }
+  return Q1_OK;
+}
+
+q1_status q1_host_columns_in_tpch_domain(q1_layout layout, const q1_columns* host, uint64_t rows, int threads,
+                                         int* in_domain) {
+  if (!host || !in_domain || (int)layout < 0 || layout > Q1_LAYOUT_COMPRESSED_PUSHDOWN) return Q1_EINVAL;
+  if (rows && (!host->quantity || !host->extended_price || !host->discount || !host->tax)) return Q1_EINVAL;
+  struct MinMax {
+    int64_t lo[4] = {INT64_MAX, INT64_MAX, INT64_MAX, INT64_MAX};
+    int64_t hi[4] = {INT64_MIN, INT64_MIN, INT64_MIN, INT64_MIN};
+  };
+  std::vector<MinMax> parts(thread_count(threads) + 1);
+  std::atomic<int> next{0};
+  try {
+    parallel_ranges(rows, 1, threads, [&](uint64_t b, uint64_t e) {
+      MinMax m;
+      for (uint64_t i = b; i < e; i++) {
+        int64_t v[4];
+        if (layout == Q1_LAYOUT_REFERENCE) {
+          v[0] = ((const int64_t*)host->discount)[i];
+          v[1] = ((const int64_t*)host->tax)[i];
+          v[2] = ((const int64_t*)host->quantity)[i];
+          v[3] = ((const int64_t*)host->extended_price)[i];
+        } else if (layout == Q1_LAYOUT_UNCOMPRESSED) {
+          v[0] = ((const int8_t*)host->discount)[i];
+          v[1] = ((const int8_t*)host->tax)[i];
+          v[2] = ((const int32_t*)host->quantity)[i];
+          v[3] = ((const int64_t*)host->extended_price)[i];
+        } else {
+          v[0] = ((const uint8_t*)host->discount)[i];
+          v[1] = ((const uint8_t*)host->tax)[i];
+          v[2] = ((const uint8_t*)host->quantity)[i];
+          v[3] = ((const uint32_t*)host->extended_price)[i];
+        }
+        for (int k = 0; k < 4; k++) {
+          m.lo[k] = v[k] < m.lo[k] ? v[k] : m.lo[k];
+          m.hi[k] = v[k] > m.hi[k] ? v[k] : m.hi[k];
+        }
+      }
+      parts[next++] = m;
+    });
+  } catch (...) {
+    return Q1_ENOMEM;
+  }
+  MinMax all;
+  for (const MinMax& m : parts)
+    for (int k = 0; k < 4; k++) {
+      all.lo[k] = m.lo[k] < all.lo[k] ? m.lo[k] : all.lo[k];
+      all.hi[k] = m.hi[k] > all.hi[k] ? m.hi[k] : all.hi[k];
+    }
+  const q1::Domain d{all.lo[0], all.hi[0], all.lo[1], all.hi[1], all.lo[2], all.hi[2], all.lo[3], all.hi[3]};
+  *in_domain = rows == 0 || q1::domain_is_tpch(layout, d);
   return Q1_OK;
 }
 

@@ -40,6 +40,9 @@ struct q1_ctx {
   std::string error;
   int32_t compute_units = 0;
   q1_launch tuning[4][5]{};                // [layout][placement]
+  // The batched kernel's tuned shape per [layout][placement]: num_blocks holds blocks per
+  // compute unit, tuples_per_thread rows per lane; zero = not tuned.
+  q1_launch batch_tuning[4][5]{};
   q1_placement best[4] = {Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS,
                           Q1_PLACE_IN_REGISTERS};
   q1_batch_state* batch = nullptr;

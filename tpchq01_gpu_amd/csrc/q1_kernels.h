@@ -99,5 +99,8 @@ uint32_t batch_max_lane_rows(q1_placement p);
 uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p);
 q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* dev_buffer,
                        q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream);
+// The host function launch_batch launches for (layout, placement), e.g. to find its node in
+// a captured graph.
+const void* batch_kernel_function(q1_layout layout, q1_placement placement);
 
 }  // namespace q1

@@ -1,16 +1,25 @@
 // Serving: continuous batching of Q1 requests over HBM-resident tables.
 //
-//   submit() -> pending queue (bounded: Q1_EBUSY when full)
+//   submit(): on an idle server (nothing queued, a slot free, the request admissible) the
+//     submitting thread launches the request at once as a batch of one; otherwise it
+//     joins the pending queue (bounded: Q1_EBUSY when full).
 //   scheduler thread: whenever an in-flight slot is free, takes up to max_batch pending
-//     requests of one layout (waiting at most batch_window_us for more while other batches
-//     are still running), admits them while the device memory of in-flight request state
-//     stays within memory_budget, and launches them as one batch on the slot's stream
-//     (a hipGraph per slot: descriptor copy, zero fill kernel, batched kernel, result copy)
-//   one completion thread per slot: waits for the slot's batch, publishes every result
-//     with its latency, frees the slot.
-// Batches on different slots run concurrently on the device, so requests that arrive while
-// others are on the GPU start as soon as a slot frees, not when the device drains.
+//     requests of one layout (optionally holding them back batch_window_us for more while
+//     other batches are still running), admits them while the device memory the
+//     in-flight work pins stays within memory_budget, and launches them as one batch on
+//     the slot's stream (a hipGraph per slot: descriptor copy, zero fill kernel, batched
+//     kernel, result copy).
+//   completion: a thread per slot waits for the slot's batch and publishes every result
+//     with its latency; a client waiting for a request of a launched batch waits for the
+//     device itself and publishes the batch first if it gets there first.
+// The batched kernel of a served batch runs one block per block-sized entry of its
+// requests (not a persistent grid), so the blocks of batches on different slots
+// interleave on the device: a small request that arrives while a large one runs starts at
+// once on a free slot and does not wait for the large one to drain.
+// Admission by device memory: a request pins its table's columns (counted once per table
+// while any of its requests is in flight) and its descriptor and result table.
 // Requests on tables outside the TPC-H value domain run alone through the 64-bit kernel.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -37,7 +46,21 @@ struct Finished {
   q1_request_stats stats{};
 };
 
+struct Where {
+  size_t slot;
+  uint32_t batch_id;
+};
+
 uint64_t request_state_bytes() { return sizeof(q1::BatchDesc) + sizeof(q1_device_aggregates); }
+
+// Device bytes of a table's columns (what a request on it keeps resident).
+uint64_t table_bytes(const q1_table* t) {
+  uint64_t b[8] = {};
+  if (q1_layout_column_bytes(t->layout, t->rows, b) != Q1_OK) return 0;
+  uint64_t sum = 0;
+  for (uint64_t x : b) sum += x;
+  return sum;
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -51,6 +74,8 @@ struct Slot {
   q1_device_aggregates* d_out = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraphNode_t kernel_node = nullptr;  // the batched kernel's node (its grid is set per batch)
+  hipKernelNodeParams kernel_params{};
   std::vector<Pending> batch;
   Clock::time_point started;
   uint32_t batch_id = 0;
@@ -69,7 +94,8 @@ struct q1_server {
   std::condition_variable cv_sched, cv_done, cv_slot;
   std::deque<Pending> queue;
   std::unordered_map<uint64_t, Finished> finished;
-  std::unordered_map<uint64_t, bool> inflight;  // tickets on the device
+  std::unordered_map<uint64_t, Where> inflight;            // tickets on the device
+  std::unordered_map<const q1_table*, uint32_t> pinned;    // in-flight requests per table
   uint64_t next_ticket = 1;
   uint64_t bytes_in_flight = 0;
   uint32_t batches = 0;
@@ -89,6 +115,7 @@ void free_slot_buffers(Slot& s) {
   if (s.d_out) (void)hipFree(s.d_out);
   s.exec = nullptr;
   s.graph = nullptr;
+  s.kernel_node = nullptr;
   s.h_buf = s.d_buf = nullptr;
   s.h_out = s.d_out = nullptr;
   s.capacity = 0;
@@ -128,12 +155,30 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
     free_slot_buffers(s);
     return Q1_EDEVICE;
   }
+  // The batched kernel's node, whose grid each launch sets to the batch's entries.  If it
+  // cannot be found the captured (resident) grid stays.
+  size_t count = 0;
+  if (hipGraphGetNodes(g, nullptr, &count) == hipSuccess && count) {
+    std::vector<hipGraphNode_t> nodes(count);
+    if (hipGraphGetNodes(g, nodes.data(), &count) == hipSuccess) {
+      const void* fn = q1::batch_kernel_function(layout, sv->placement);
+      for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType type;
+        hipKernelNodeParams kp{};
+        if (hipGraphNodeGetType(node, &type) == hipSuccess && type == hipGraphNodeTypeKernel &&
+            hipGraphKernelNodeGetParams(node, &kp) == hipSuccess && kp.func == fn) {
+          s.kernel_node = node;
+          s.kernel_params = kp;
+        }
+      }
+    }
+  }
   s.capacity = cap;
   s.layout = layout;
   return Q1_OK;
 }
 
-// Launches the slot's batch; called with the server lock released.
+// Launches the slot's batch; called with the server lock released, on the server's device.
 q1_status launch_slot(q1_server* sv, Slot& s) {
   const uint32_t n = (uint32_t)s.batch.size();
   const q1_table* t0 = s.batch[0].req.table;
@@ -159,7 +204,10 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
     if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,
                                       s.stream) != hipSuccess)
       st = Q1_EDEVICE;
-    if (st == Q1_OK && hipEventRecord(s.done, s.stream) != hipSuccess) st = Q1_EDEVICE;
+    if (st == Q1_OK && hipEventRecord(s.done, s.stream) != hipSuccess) {
+      (void)hipStreamSynchronize(s.stream);
+      st = Q1_EDEVICE;
+    }
     return st;
   }
   uint32_t cap = 16;
@@ -175,14 +223,40 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
   s.h_buf->total_blocks = entries;
   s.h_buf->tiles_per_thread = lane_rows / q1::layout_tile_rows(t0->layout);
   s.h_buf->claimed = 0;
-  if (hipGraphLaunch(s.exec, s.stream) != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess)
+  if (s.kernel_node) {
+    // one block per entry: block b runs entry b, its claim then lies past the list
+    hipKernelNodeParams kp = s.kernel_params;
+    kp.gridDim = dim3((uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, 1), (1u << 24) - 1));
+    if (hipGraphExecKernelNodeSetParams(s.exec, s.kernel_node, &kp) != hipSuccess) s.kernel_node = nullptr;
+  }
+  if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return Q1_EDEVICE;
+  if (hipEventRecord(s.done, s.stream) != hipSuccess) {
+    // The graph may still be reading the slot's pinned buffers: let it finish before the
+    // caller frees the slot for the next batch.
+    (void)hipStreamSynchronize(s.stream);
     return Q1_EDEVICE;
+  }
   return Q1_OK;
 }
 
-void publish(q1_server* sv, Slot& s, q1_status st) {
+// Device bytes admitting `r` would add to what the in-flight work pins.
+uint64_t admission_cost(const q1_server* sv, const q1_request& r) {
+  return request_state_bytes() + (sv->pinned.count(r.table) ? 0 : table_bytes(r.table));
+}
+
+// Puts a pending request into a slot's batch (server lock held).
+void take(q1_server* sv, size_t slot, const Pending& p) {
+  sv->bytes_in_flight += admission_cost(sv, p.req);
+  sv->pinned[p.req.table]++;
+  sv->inflight[p.ticket] = Where{slot, sv->slots[slot].batch_id};
+  sv->slots[slot].batch.push_back(p);
+}
+
+// Publishes the results of batch `id` of the slot, unless they were published already.
+void publish(q1_server* sv, Slot& s, q1_status st, uint32_t id) {
   const auto now = Clock::now();
   std::lock_guard<std::mutex> g(sv->m);
+  if (!s.busy || s.batch_id != id) return;
   for (size_t i = 0; i < s.batch.size(); i++) {
     Finished f;
     f.status = st;
@@ -194,8 +268,13 @@ void publish(q1_server* sv, Slot& s, q1_status st) {
     f.stats.batch_id = s.batch_id;
     sv->finished[s.batch[i].ticket] = f;
     sv->inflight.erase(s.batch[i].ticket);
+    const q1_table* t = s.batch[i].req.table;
+    sv->bytes_in_flight -= request_state_bytes();
+    if (--sv->pinned[t] == 0) {
+      sv->pinned.erase(t);
+      sv->bytes_in_flight -= table_bytes(t);
+    }
   }
-  sv->bytes_in_flight -= s.batch.size() * request_state_bytes();
   s.batch.clear();
   s.busy = false;
   s.launched = false;
@@ -207,14 +286,31 @@ void slot_waiter(q1_server* sv, size_t idx) {
   Slot& s = sv->slots[idx];
   (void)hipSetDevice(sv->ctx->device);
   for (;;) {
+    uint32_t id;
     {
       std::unique_lock<std::mutex> lk(sv->m);
       sv->cv_slot.wait(lk, [&] { return (sv->stop && !s.busy) || (s.busy && s.launched); });
       if (!s.busy) return;
+      id = s.batch_id;
     }
     const q1_status st = hipEventSynchronize(s.done) == hipSuccess ? Q1_OK : Q1_EDEVICE;
-    publish(sv, s, st);
+    publish(sv, s, st, id);
   }
+}
+
+// Launches the batch just put into `slot` (server lock held on entry and on return).
+void launch_taken(q1_server* sv, Slot& slot, std::unique_lock<std::mutex>& lk) {
+  const uint32_t id = slot.batch_id;
+  lk.unlock();
+  const q1_status st = launch_slot(sv, slot);
+  if (st != Q1_OK) {
+    publish(sv, slot, st, id);
+    lk.lock();
+    return;
+  }
+  lk.lock();
+  slot.launched = true;
+  sv->cv_slot.notify_all();
 }
 
 void scheduler_loop(q1_server* sv) {
@@ -240,7 +336,7 @@ void scheduler_loop(q1_server* sv) {
       sv->cv_done.notify_all();
       return;
     }
-    // Continuous batching: while other batches run, give the queue a short window to fill.
+    // Optionally, while other batches run, give the queue a short window to fill.
     bool others_busy = false;
     for (auto& s : sv->slots) others_busy = others_busy || s.busy;
     if (others_busy && sv->queue.size() < sv->cfg.max_batch && sv->cfg.batch_window_us) {
@@ -248,10 +344,12 @@ void scheduler_loop(q1_server* sv) {
                             [&] { return sv->stop || sv->queue.size() >= sv->cfg.max_batch; });
       if (sv->stop) continue;
     }
-    Slot* slot = nullptr;
-    for (auto& s : sv->slots)
-      if (!s.busy) slot = &s;
-    if (!slot || sv->queue.empty()) continue;
+    size_t idx = sv->slots.size();
+    for (size_t i = 0; i < sv->slots.size(); i++)
+      if (!sv->slots[i].busy) idx = i;
+    if (idx == sv->slots.size() || sv->queue.empty()) continue;
+    Slot* slot = &sv->slots[idx];
+    slot->batch_id = ++sv->batches;
     // Admission: requests of the front request's layout, within the memory budget.
     const q1_table* t0 = sv->queue.front().req.table;
     slot->single = !t0->fast;
@@ -261,15 +359,13 @@ void scheduler_loop(q1_server* sv) {
       Pending p = sv->queue.front();
       sv->queue.pop_front();
       const bool same = p.req.table->layout == t0->layout && p.req.table->fast == t0->fast;
-      const bool fits = sv->bytes_in_flight + request_state_bytes() <= sv->cfg.memory_budget;
+      const bool fits = sv->bytes_in_flight + admission_cost(sv, p.req) <= sv->cfg.memory_budget;
       if (!same || !fits) {
         rest.push_back(p);
         if (!fits) break;
         continue;
       }
-      sv->bytes_in_flight += request_state_bytes();
-      sv->inflight[p.ticket] = true;
-      slot->batch.push_back(p);
+      take(sv, idx, p);
     }
     while (!rest.empty()) {
       sv->queue.push_front(rest.back());
@@ -282,17 +378,7 @@ void scheduler_loop(q1_server* sv) {
     }
     slot->busy = true;
     slot->started = Clock::now();
-    slot->batch_id = ++sv->batches;
-    lk.unlock();
-    const q1_status st = launch_slot(sv, *slot);
-    if (st != Q1_OK) {
-      publish(sv, *slot, st);
-      lk.lock();
-      continue;
-    }
-    lk.lock();
-    slot->launched = true;
-    sv->cv_slot.notify_all();
+    launch_taken(sv, *slot, lk);
   }
 }
 
@@ -303,23 +389,29 @@ extern "C" {
 q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server** out) {
   if (!ctx || !out) return Q1_EINVAL;
   *out = nullptr;
-  q1_server_config c{256, 2, 65536, 200, 64ull << 20};
+  q1_server_config c{256, 2, 65536, 0, 0};
   if (cfg) {
     if (cfg->max_batch) c.max_batch = cfg->max_batch;
     if (cfg->max_inflight) c.max_inflight = cfg->max_inflight;
     if (cfg->queue_capacity) c.queue_capacity = cfg->queue_capacity;
     c.batch_window_us = cfg->batch_window_us;
-    if (cfg->memory_budget) c.memory_budget = cfg->memory_budget;
+    c.memory_budget = cfg->memory_budget;
   }
   if (c.max_batch > 4096 || c.max_inflight > 8) return q1::fail(ctx, Q1_EINVAL, "max_batch <= 4096, max_inflight <= 8");
   try {
+    (void)hipSetDevice(ctx->device);
+    if (!c.memory_budget) {
+      // default: three quarters of the device's memory
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return q1::fail(ctx, Q1_EDEVICE, "hipMemGetInfo failed");
+      c.memory_budget = (uint64_t)total_b / 4 * 3;
+    }
     q1_server* sv = new q1_server();
     sv->ctx = ctx;
     sv->cfg = c;
     const q1_placement best = ctx->best[Q1_LAYOUT_COMPRESSED];
     sv->placement = best == Q1_PLACE_PER_THREAD_SHARED_MEM ? best : Q1_PLACE_IN_REGISTERS;
     sv->slots.resize(c.max_inflight);
-    (void)hipSetDevice(ctx->device);
     for (auto& s : sv->slots) {
       if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
           hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
@@ -340,12 +432,33 @@ q1_status q1_server_submit(q1_server* sv, const q1_request* req, uint64_t* ticke
   if (!sv || !req || !ticket || !req->table) return Q1_EINVAL;
   const q1_table* t = req->table;
   if (req->row_begin > t->rows || req->row_count > t->rows - req->row_begin) return Q1_EINVAL;
-  if (request_state_bytes() > sv->cfg.memory_budget) return Q1_ENOMEM;  // could never be admitted
-  std::lock_guard<std::mutex> g(sv->m);
+  // could never be admitted: its table and state alone exceed the budget
+  if (request_state_bytes() + table_bytes(t) > sv->cfg.memory_budget) return Q1_ENOMEM;
+  std::unique_lock<std::mutex> lk(sv->m);
   if (sv->stop) return Q1_ECANCELED;
   if (sv->queue.size() >= sv->cfg.queue_capacity) return Q1_EBUSY;
   Pending p{sv->next_ticket++, *req, Clock::now()};
   *ticket = p.ticket;
+  // An idle server launches the request from this thread, with no hand-off to the
+  // scheduler: nothing is queued ahead of it, a slot is free and it is admissible.
+  size_t idx = sv->slots.size();
+  if (sv->queue.empty())
+    for (size_t i = 0; i < sv->slots.size(); i++)
+      if (!sv->slots[i].busy) idx = i;
+  if (idx < sv->slots.size() && sv->bytes_in_flight + admission_cost(sv, *req) <= sv->cfg.memory_budget) {
+    Slot& slot = sv->slots[idx];
+    slot.batch_id = ++sv->batches;
+    slot.single = !t->fast;
+    take(sv, idx, p);
+    slot.busy = true;
+    slot.started = p.submitted;
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != sv->ctx->device) (void)hipSetDevice(sv->ctx->device);
+    launch_taken(sv, slot, lk);
+    if (prev >= 0 && prev != sv->ctx->device) (void)hipSetDevice(prev);
+    return Q1_OK;   // a failed launch reaches the client with the request's result
+  }
   sv->queue.push_back(p);
   sv->cv_sched.notify_all();
   return Q1_OK;
@@ -359,10 +472,27 @@ q1_status q1_server_wait(q1_server* sv, uint64_t ticket, int64_t timeout_us, q1_
   if (!known)
     for (auto& p : sv->queue) known = known || p.ticket == ticket;
   if (!known) return Q1_EINVAL;
-  if (timeout_us < 0) {
-    sv->cv_done.wait(lk, ready);
-  } else if (!sv->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), ready)) {
-    return Q1_ETIMEOUT;
+  const auto deadline = Clock::now() + std::chrono::microseconds(timeout_us < 0 ? 0 : timeout_us);
+  while (!ready()) {
+    auto it = sv->inflight.find(ticket);
+    if (timeout_us < 0 && it != sv->inflight.end()) {
+      Slot& s = sv->slots[it->second.slot];
+      const uint32_t id = it->second.batch_id;
+      if (s.busy && s.launched && s.batch_id == id) {
+        // the request's batch is on the device: wait for the device here rather than for
+        // the completion thread's wake-up, and publish the batch if first
+        lk.unlock();
+        const q1_status st = hipEventSynchronize(s.done) == hipSuccess ? Q1_OK : Q1_EDEVICE;
+        publish(sv, s, st, id);
+        lk.lock();
+        continue;
+      }
+    }
+    if (timeout_us < 0) {
+      sv->cv_done.wait(lk);
+    } else if (sv->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !ready()) {
+      return Q1_ETIMEOUT;
+    }
   }
   Finished f = sv->finished[ticket];
   sv->finished.erase(ticket);

@@ -14,7 +14,10 @@ from . import q1
 
 class Server:
     def __init__(self, ctx: q1.Context, max_batch=256, max_inflight=2, queue_capacity=65536,
-                 batch_window_us=200, memory_budget=64 << 20):
+                 batch_window_us=0, memory_budget=0):
+        """memory_budget: device bytes the in-flight work may pin (the columns of each table
+        with requests in flight, plus descriptor and result state per request); 0 = three
+        quarters of the device's memory."""
         if not hasattr(q1.lib, "q1_server_create"):
             raise RuntimeError("libq1hip.so was built without the server")
         self.ctx = ctx
