@@ -216,6 +216,19 @@ q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout,
                      const q1_launch* launch, q1_device_aggregates* aggregates,
                      const q1_columns* device, uint64_t row_begin, uint64_t num_tuples,
                      int32_t cutoff_day, void* hip_stream);
+/* No counterpart in the reference: q1_enqueue that allocates nothing (no hipMalloc, hipFree
+ * or allocating library call), for callers that cannot allocate device memory -- code
+ * under HIP graph capture, serving loops that forbid allocation.  Same arguments, results
+ * and errors as q1_enqueue.  Its scratch (the spread-merge tables of grids of 1024 blocks or
+ * more, and the global placement's per-block tables) is a static array of 8 MiB + 24 KiB
+ * per GPU, set up by q1_init and held until the process ends; a launch needing more (the
+ * global placement with more than 34952 blocks) gets Q1_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams or threads) race, so the
+ * caller orders them.  The global placement may run on any stream here. */
+q1_status q1_enqueue_noalloc(q1_ctx* ctx, q1_placement placement, q1_layout layout,
+                             const q1_launch* launch, q1_device_aggregates* aggregates,
+                             const q1_columns* device, uint64_t row_begin, uint64_t num_tuples,
+                             int32_t cutoff_day, void* hip_stream);
 /* Converts a host copy of the device table into a result. */
 void q1_aggregates_to_result(const q1_device_aggregates* host_copy, q1_result* out);
 /* Exact 128-bit merge of two results (out = a + b). */

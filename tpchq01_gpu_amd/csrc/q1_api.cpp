@@ -251,6 +251,8 @@ q1_status q1_init(int device, q1_ctx** out) {
   ok = ok && hipMalloc(&ctx->d_stats, 8 * sizeof(int64_t)) == hipSuccess;
   ok = ok && hipMalloc(&ctx->spread.tables, kSpreadTables * sizeof(q1_device_aggregates)) == hipSuccess;
   ok = ok && hipMemset(ctx->spread.tables, 0, kSpreadTables * sizeof(q1_device_aggregates)) == hipSuccess;
+  // the scratch of q1_enqueue_noalloc on this device (once per device and process)
+  ok = ok && noalloc_setup(device, ctx->stream) == Q1_OK;
   ok = ok && hipDeviceSynchronize() == hipSuccess;
   if (!ok) {
     q1_shutdown(ctx);
@@ -656,6 +658,41 @@ q1_status q1_last_timing(q1_ctx* ctx, q1_timing* out) {
   if (!ctx || !out) return Q1_EINVAL;
   *out = ctx->last;
   return Q1_OK;
+}
+
+q1_status q1_enqueue_noalloc(q1_ctx* ctx, q1_placement placement, q1_layout layout, const q1_launch* launch,
+                             q1_device_aggregates* aggregates, const q1_columns* device, uint64_t row_begin,
+                             uint64_t num_tuples, int32_t cutoff_day, void* hip_stream) {
+  if (!ctx || !aggregates || !device || !valid_layout(layout) || !valid_placement(placement))
+    return fail(ctx, Q1_EINVAL, "bad argument");
+  GUARD_BEGIN
+  std::lock_guard<std::mutex> g(ctx->lock);
+  uint64_t b[8];
+  column_bytes(layout, row_begin + num_tuples, b);
+  for (int i = 0; i < 8; i++) {
+    if (!b[i]) continue;
+    const void* p = colc(*device, i);
+    if (!p) return fail(ctx, Q1_EINVAL, "a column required by the layout is NULL");
+    if ((uintptr_t)p % 16) return fail(ctx, Q1_EINVAL, "device columns must be 16-byte aligned");
+  }
+  const q1_placement p = resolve_placement(ctx, layout, placement);
+  q1_launch l;
+  resolve_launch(ctx, layout, p, num_tuples, launch, &l, true);
+  LaunchShape shape;
+  const char* why = "";
+  q1_status s = make_shape(p, layout, &l, row_begin, row_begin + num_tuples, &shape, &why);
+  if (s != Q1_OK) return fail(ctx, s, why);
+  ScanArgs a;
+  a.cols = *device;
+  a.row_begin = row_begin;
+  a.row_end = row_begin + num_tuples;
+  a.cutoff_day = cutoff_day;
+  a.out = aggregates;
+  // the device's static scratch array instead of the context's spread tables and workspace
+  s = noalloc_launch_scan(ctx->device, p, layout, (l.flags & Q1_LAUNCH_ASSUME_TPCH_DOMAIN) != 0, a, shape,
+                          hip_stream ? hip_stream : (void*)ctx->stream, &why);
+  return s == Q1_OK ? Q1_OK : fail(ctx, s, why);
+  GUARD_END(ctx)
 }
 
 q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout, const q1_launch* launch,

@@ -103,4 +103,14 @@ q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* de
 // a captured graph.
 const void* batch_kernel_function(q1_layout layout, q1_placement placement);
 
+// The scratch of q1_enqueue_noalloc (q1_noalloc.hip): a static per-device array holding the
+// spread tables and kNoallocWorkspaceBytes of per-block tables for the global placement.
+// noalloc_setup looks up the array's address on `device` (which must be current) and
+// clears its spread tables, once per device; noalloc_launch_scan is launch_scan with the
+// array as its scratch, refusing launches that need more (*why says which).
+constexpr uint64_t kNoallocWorkspaceBytes = 8ull << 20;
+q1_status noalloc_setup(int device, void* stream);
+q1_status noalloc_launch_scan(int device, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
+                              const LaunchShape& shape, void* stream, const char** why);
+
 }  // namespace q1

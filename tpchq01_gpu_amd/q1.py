@@ -166,6 +166,9 @@ def _load():
         "q1_run": (I, [P, P, U64, U64, I32, I, C.POINTER(Launch), C.POINTER(Result)]),
         "q1_last_timing": (I, [P, C.POINTER(Timing)]),
         "q1_enqueue": (I, [P, I, I, C.POINTER(Launch), P, C.POINTER(Columns), U64, U64, I32, P]),
+        "q1_enqueue_noalloc": (I, [P, I, I, C.POINTER(Launch), P, C.POINTER(Columns), U64, U64, I32, P]),
+        "q1_table_refresh_domain": (I, [P, P]),
+        "q1_host_columns_in_tpch_domain": (I, [I, C.POINTER(Columns), U64, I, C.POINTER(C.c_int)]),
         "q1_aggregates_to_result": (None, [C.POINTER(DeviceAggregates), C.POINTER(Result)]),
         "q1_result_merge": (None, [C.POINTER(Result), C.POINTER(Result), C.POINTER(Result)]),
         "q1_host_alloc": (P, [P, C.c_size_t]),
