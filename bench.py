@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-stream-probe", dest="stream_probe", action="store_false")
     ap.add_argument("--no-layouts", dest="layouts", action="store_false",
                     help="skip the pushdown / uncompressed / reference layouts leg")
+    ap.add_argument("--no-placements", dest="placements", action="store_false",
+                    help="skip the leg that runs the headline table through every placement")
     ap.add_argument("--no-host", dest="host", action="store_false",
                     help="skip the host-resident streaming and CPU/GPU split leg")
     ap.add_argument("--serve-plan", default="2000:60,8000:5",
@@ -178,12 +180,14 @@ def hbm_probe(rows: int):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def batch_leg(q1, ctx, spec):
+def batch_leg(q1, ctx, spec, headline=None):
     """Batch leg: len(spec) requests, each on its own seeded column slice (log-uniform
     1e4..2e7 rows, cutoff uniform in 1998-06-01..1998-11-30), in one q1_run_batch
     (persistent kernel replayed from a hipGraph) vs the plain per-request q1_run one at a
-    time.  Returns the leg's figures and the batch's results (checked against the oracle
-    later, in the CPU baseline leg)."""
+    time.  With `headline` (table, rows) it also runs that whole table as a batch of one
+    request beside q1_run, which isolates the batched kernel's own cost.  Returns the leg's
+    figures and the batch's results (checked against the oracle later, in the CPU baseline
+    leg)."""
     import numpy as np
     n = len(spec)
     sizes = np.array([s[1] for s in spec], dtype=np.int64)
@@ -194,11 +198,31 @@ def batch_leg(q1, ctx, spec):
     for _ in range(2):   # build the graph, then warm
         ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
     reps = 5
+    dev = []
     t0 = time.perf_counter()
     for _ in range(reps):
         ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
+        dev.append(ctx.last_timing().kernel_ms)
     batched = (time.perf_counter() - t0) / reps
-    batched_dev = ctx.last_timing().kernel_ms * 1e-3
+    batched_dev = sum(dev) / len(dev) * 1e-3
+    tm = ctx.last_timing()
+    shape = {"blocks": tm.num_blocks, "tuples_per_thread": tm.tuples_per_thread,
+             "placement": q1.PLACEMENT_NAMES[tm.placement]}
+    one_table = None
+    if headline is not None:
+        table, trows = headline
+        req1 = (q1.Request * 1)(q1.Request(table.handle, 0, trows, 729999, 0))
+        out1 = (q1.Result * 1)()
+        bms, sms = [], []
+        for rep in range(6):
+            ctx.check(q1.lib.q1_run_batch(ctx.handle, req1, 1, None, out1), "q1_run_batch")
+            b = ctx.last_timing().kernel_ms
+            ctx.run_raw(table, 0, trows, 729999, q1.AUTO)
+            if rep:
+                bms.append(b)
+                sms.append(ctx.last_timing().kernel_ms)
+        one_table = {"rows": trows, "batch_kernel_ms": sum(bms) / len(bms), "scan_kernel_ms": sum(sms) / len(sms)}
+        # the 256-request graph is rebuilt on the next call of that size
     singles = []
     for r in range(n):   # warm the per-request path
         ctx.run_raw(tables[r], 0, int(sizes[r]), int(cutoffs[r]), q1.AUTO)
@@ -215,7 +239,7 @@ def batch_leg(q1, ctx, spec):
             "batched_req_s": n / batched, "batched_rows_s": rows / batched,
             "batched_device_gbps": rows * q1.BITS_PER_ROW[q1.COMPRESSED] / 8 / batched_dev / 1e9,
             "one_at_a_time_ms": one * 1e3, "one_at_a_time_req_s": n / one,
-            "speedup": one / batched, "exact_vs_single": exact}, results
+            "speedup": one / batched, "exact_vs_single": exact, "shape": shape, "one_table": one_table}, results
 
 
 def probe_ceiling(q1, probe, layout):
@@ -278,6 +302,27 @@ def layouts_leg(q1, ctx, comp_table, rows: int, seed: int, reps: int = 5, probe=
 def as_dicts(groups):
     return [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
                  sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in groups]
+
+
+def placements_leg(q1, ctx, table, rows: int, want, reps: int = 10):
+    """The headline table (compressed, device-resident) through every placement with its
+    tuned launch: kernel time of q1_run (mean of `reps` after one warm-up), HBM fraction,
+    and each placement's result against the headline result."""
+    algo = rows * q1.BITS_PER_ROW[table.layout] // 8
+    out = {}
+    for p in (q1.IN_REGISTERS, q1.IN_REGISTERS_PER_THREAD, q1.LOCAL_MEM, q1.PER_THREAD_SHARED_MEM, q1.GLOBAL):
+        got = q1.result_to_list(ctx.run_raw(table, 0, rows, 729999, p))
+        ms = []
+        for _ in range(reps):
+            ctx.run_raw(table, 0, rows, 729999, p)
+            ms.append(ctx.last_timing().kernel_ms)
+        tm = ctx.last_timing()
+        ms = sum(ms) / len(ms)
+        out[q1.PLACEMENT_NAMES[p]] = {"kernel_ms": ms, "gbps": algo / (ms * 1e-3) / 1e9,
+                                      "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "rows_s": rows / (ms * 1e-3),
+                                      "threads_per_block": tm.threads_per_block,
+                                      "tuples_per_thread": tm.tuples_per_thread, "exact_vs_headline": got == want}
+    return out
 
 
 def uncompressed_leg(q1, ctx, rows: int, seed: int, reps: int = 20):
@@ -566,6 +611,10 @@ def main():
             "frac": algo_bytes / (wms * 1e-3) / 1e9 / HBM_PEAK_GBS, "threads_per_block": 256,
             "tuples_per_thread": 1024, "exact_vs_fast": exact}
         log(f"bench: layouts {out['layouts']}")
+    if args.placements:
+        log("bench: the headline table through every placement")
+        out["placements"] = placements_leg(q1, ctx, table, rows, one)
+        log(f"bench: placements {out['placements']}")
     if args.layouts and layout == q1.COMPRESSED:
         log(f"bench: {args.cpu_rows} uncompressed rows through every placement")
         out["uncompressed_60M"] = uncompressed_leg(q1, ctx, args.cpu_rows, 0x51C00000)
@@ -579,7 +628,8 @@ def main():
         from tpchq01_gpu_amd import workloads
         spec = workloads.batch_spec(args.batch_requests)
         log(f"bench: batch leg, {args.batch_requests} requests")
-        out["batch"], batch_results = batch_leg(q1, ctx, spec)
+        out["batch"], batch_results = batch_leg(q1, ctx, spec,
+                                                (table, rows) if layout == q1.COMPRESSED else None)
         log(f"bench: batch {out['batch']}")
     plan = [(float(r), float(s)) for r, s in (x.split(":") for x in args.serve_plan.split(",") if x)]
     if plan:

@@ -990,26 +990,40 @@ struct AccPerThread {
 // Q1_PLACE_LOCAL_MEM: the per-thread table of the reference
 // (src/kernels/ht_in_local_mem.cuh:54-59) lives in the lane's private (scratch) memory:
 // [group][count|discount|quantity, price, discount product, charge], 64-bit words.  Rows
-// are first summed per lane tile (at most 16 rows) in narrow registers with the
-// wavefront-uniform group skip of AccRegisters; each tile's sums are then added into the
-// private table, one read-modify-write per word of each group the lane touched.  A row
-// costs registers only; the table sees a sixteenth of the rows' updates.
+// are first summed over kLocalFlushRows rows of the lane in registers, with the
+// wavefront-uniform group skip of AccRegisters; those sums are then added into the private
+// table, one read-modify-write per word of each group the lane touched.  A row costs
+// registers only; the table sees one update per word and group every kLocalFlushRows rows.
+// (With an update every 16 rows the private tables of the resident wavefronts outgrew the
+// caches and the launch took 3.2 ms at 600M rows.)
+constexpr uint32_t kLocalFlushRows = 64;
 struct AccLocal {
   static constexpr bool kPackedSmall = false;
-  uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
-  uint64_t nc[kNumGroups];
-  volatile uint64_t t[kNumGroups][4];  // volatile: the table stays in private memory
-  __device__ __forceinline__ void init(char*) {
+  // register sums of at most kLocalFlushRows rows: count, discount and quantity in the
+  // pack_small64 layout (count < 2^7, discount < 2^10, quantity < 2^19), price below 2^30
+  static constexpr uint32_t kTableWords = kNumGroups * 4;
+  uint64_t ns[kNumGroups], nd[kNumGroups], nc[kNumGroups];
+  uint32_t np[kNumGroups];
+  uint32_t calls;
+  // The table (kTableWords words of the caller's private array) is used only through this
+  // pointer, which an empty asm hides from the optimizer: the array cannot become
+  // registers, so it stays in private memory, and its loads and stores are ordinary ones
+  // the compiler may overlap (volatile accesses would each wait for the previous one).
+  uint64_t* t;
+  __device__ __forceinline__ void init(char* private_table) {
+    calls = 0;
+    t = reinterpret_cast<uint64_t*>(private_table);
+    __asm__ volatile("" : "+v"(t));
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      ns[g] = np[g] = nd[g] = 0;
-      nc[g] = 0;
+      ns[g] = nd[g] = nc[g] = 0;
+      np[g] = 0;
 #pragma unroll
-      for (int f = 0; f < 4; f++) t[g][f] = 0;
+      for (int f = 0; f < 4; f++) t[g * 4 + f] = 0;
     }
   }
   __device__ __forceinline__ void add(const FastRow& r) {
-    const uint32_t s = pack_small(r);
+    const uint64_t s = pack_small64(r);
     const uint32_t dprod = disc_prod(r);
     const uint64_t c = fast_charge(r);
     const uint64_t pass_mask = ballot(r.pass);
@@ -1026,29 +1040,34 @@ struct AccLocal {
       }
     }
   }
-  // Called after at most 16 rows: adds the tile's sums into the private table.
+  // Called every 16 rows of a lane (at the same points in every lane); every
+  // kLocalFlushRows rows the register sums go into the private table.
   __device__ __forceinline__ void flush() {
+    if (++calls % (kLocalFlushRows / 16) != 0) return;
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       if (ballot(ns[g] != 0)) {
         if (ns[g] != 0) {
-          t[g][0] = t[g][0] + small_to_small64(ns[g]);
-          t[g][1] = t[g][1] + np[g];
-          t[g][2] = t[g][2] + nd[g];
-          t[g][3] = t[g][3] + nc[g];
+          uint64_t* w = t + g * 4;
+          w[0] += ns[g];
+          w[1] += np[g];
+          w[2] += nd[g];
+          w[3] += nc[g];
         }
-        ns[g] = np[g] = nd[g] = 0;
-        nc[g] = 0;
+        ns[g] = nd[g] = nc[g] = 0;
+        np[g] = 0;
       }
     }
   }
+  // The private table plus the register sums not yet added to it.
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      unpack_small64(t[g][0], o.v[g]);
-      o.v[g][F_PRICE] = t[g][1];
-      o.v[g][F_DPROD] = t[g][2];
-      o.v[g][F_CHARGE] = t[g][3];
+      const uint64_t* w = t + g * 4;
+      unpack_small64(w[0] + ns[g], o.v[g]);
+      o.v[g][F_PRICE] = w[1] + np[g];
+      o.v[g][F_DPROD] = w[2] + nd[g];
+      o.v[g][F_CHARGE] = w[3] + nc[g];
     }
   }
   static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
@@ -1165,8 +1184,11 @@ struct AccSharedPacked {
 // five device atomics; here rows are summed in the lanes' registers (AccRegisters), each
 // wavefront reduces its lanes' sums with DPP and adds them into the block's table with
 // one atomic per (group, field) from one lane, and the block then merges its table into
-// the destination once.  The table sees up to blockDim * 16384 rows: count and discount
-// share one word (count bits [0,24), discount [24,64)), the other sums have a word each.
+// the destination once: thread g reads group g's words and issues its destination atomics.
+// The table sees up to blockDim * 16384 <= 2^22 rows: count and discount share one word
+// (count bits [0,24), discount [24,64)), the other sums have a word each; every block sum
+// fits 64 bits (price < 2^46, discount product < 2^50, charge < 2^60).
+__device__ __forceinline__ void atomic_add128(uint64_t* lo_p, uint64_t* hi_p, uint64_t lo, uint64_t hi);
 struct AccGlobal {
   static constexpr bool kPackedSmall = false;
   static constexpr int W = 5;  // words per group: count|disc, qty, price, dprod, charge
@@ -1181,10 +1203,10 @@ struct AccGlobal {
   __device__ __forceinline__ void init(char* lds) { regs.init(lds); }
   __device__ __forceinline__ void add(const FastRow& r) { regs.add(r); }
   __device__ __forceinline__ void flush() { regs.flush(); }
-  // Each wavefront adds its sums into the block's table (one atomic per group and field),
-  // then thread 0 hands the whole table to the block merge; the other lanes contribute zero.
-  __device__ __forceinline__ void totals(LaneTotals& o, char* lds) const {
-    regs.totals(o, lds);
+  // The block's whole merge (called by every thread of the block).
+  __device__ __forceinline__ void merge(q1_device_aggregates* out, uint64_t qty_scale) const {
+    LaneTotals o;
+    regs.totals(o, nullptr);
     const uint32_t lane = threadIdx.x & 63;
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
@@ -1204,22 +1226,25 @@ struct AccGlobal {
         }
       }
     }
-    __threadfence();
+    // Only this block reads its table back, and every wave of the block shares one L2 (where
+    // the atomics are performed): the barrier's workgroup-scope release and acquire order
+    // them before the read-back.  A device-scope fence here would make each block wait for
+    // an L2 write-back towards the other XCDs.
     __syncthreads();
-#pragma unroll
-    for (int g = 0; g < kNumGroups; g++)
-#pragma unroll
-      for (int f = 0; f < NF; f++) o.v[g][f] = 0;
-    if (threadIdx.x == 0) {
-      for (int g = 0; g < kNumGroups; g++) {
-        auto ld = [&](int w) { return __hip_atomic_load(tab + w * kNumGroups + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
-        const uint64_t cd = ld(0);
-        o.v[g][F_CNT] = cd & ((1ull << 24) - 1);
-        o.v[g][F_DISC] = cd >> 24;
-        o.v[g][F_QTY] = ld(1);
-        o.v[g][F_PRICE] = ld(2);
-        o.v[g][F_DPROD] = ld(3);
-        o.v[g][F_CHARGE] = ld(4);
+    if (threadIdx.x < kNumGroups) {
+      const int g = threadIdx.x;
+      auto ld = [&](int w) {
+        return __hip_atomic_load(tab + w * kNumGroups + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      };
+      const uint64_t cd = ld(0), q = ld(1), p = ld(2), d = ld(3), c = ld(4);
+      if (cd) {
+        atomicAdd((unsigned long long*)&out->sum_qty[g], (unsigned long long)(q * qty_scale));
+        atomicAdd((unsigned long long*)&out->sum_base_price[g], (unsigned long long)p);
+        atomicAdd((unsigned long long*)&out->sum_disc[g], (unsigned long long)(cd >> 24));
+        atomicAdd((unsigned long long*)&out->count[g], (unsigned long long)(cd & ((1ull << 24) - 1)));
+        // sum_disc_price = 100 * sum_price - sum(disc * price) >= 0 (every disc <= 15)
+        atomic_add128(&out->sum_disc_price_lo[g], &out->sum_disc_price_hi[g], 100 * p - d, 0);
+        atomic_add128(&out->sum_charge_lo[g], &out->sum_charge_hi[g], c, 0);
       }
     }
   }
@@ -1404,8 +1429,13 @@ __device__ __forceinline__ void scan_block(const ScanArgs& a, uint64_t block, ui
   const uint64_t base = first_tile + block * B * tiles_per_thread;
 
   Acc acc;
+  // the local placement's table in the lane's private memory (unused by the others)
+  uint64_t private_table[P == Q1_PLACE_LOCAL_MEM ? AccLocal::kTableWords : 1];
   if constexpr (P == Q1_PLACE_GLOBAL) acc.init_global(ws);
-  acc.init(reinterpret_cast<char*>(lds));
+  if constexpr (P == Q1_PLACE_LOCAL_MEM)
+    acc.init(reinterpret_cast<char*>(private_table));
+  else
+    acc.init(reinterpret_cast<char*>(lds));
 
   constexpr bool kSplit = P == Q1_PLACE_PER_THREAD_SHARED_MEM || P == kPlaceSharedNarrow || P == kPlaceSharedPacked;
   if constexpr (kSplit) {
@@ -1479,18 +1509,19 @@ __device__ __forceinline__ void scan_block(const ScanArgs& a, uint64_t block, ui
   }
   acc.flush();
 
-  if constexpr (P == Q1_PLACE_GLOBAL) {
-    __threadfence();
-    __syncthreads();
-  }
-  LaneTotals tot;
-  acc.totals(tot, reinterpret_cast<char*>(lds));
-  // The reduction scratch follows the placement's own LDS table.
-  uint64_t* red = lds + Acc::lds_bytes(B) / 8;
-  __syncthreads();
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
-  block_merge<Acc::kPackedSmall>(tot, red, out, qty_scale);
+  if constexpr (P == Q1_PLACE_GLOBAL) {
+    // through the block's table in global memory
+    acc.merge(out, qty_scale);
+  } else {
+    LaneTotals tot;
+    acc.totals(tot, reinterpret_cast<char*>(lds));
+    // The reduction scratch follows the placement's own LDS table.
+    uint64_t* red = lds + Acc::lds_bytes(B) / 8;
+    __syncthreads();
+    block_merge<Acc::kPackedSmall>(tot, red, out, qty_scale);
+  }
 }
 
 template <int L, int P, int B>

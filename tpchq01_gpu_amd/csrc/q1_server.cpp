@@ -15,7 +15,9 @@
 // The batched kernel of a served batch runs one block per block-sized entry of its
 // requests (not a persistent grid), so the blocks of batches on different slots
 // interleave on the device: a small request that arrives while a large one runs starts at
-// once on a free slot and does not wait for the large one to drain.
+// once on a free slot and does not wait for the large one to drain.  A batch of one
+// request runs q1_run's scan kernel with its tuned launch instead, merging through the
+// slot's own spread tables.
 // Admission by device memory: a request pins its table's columns (counted once per table
 // while any of its requests is in flight) and its descriptor and result table.
 // Requests on tables outside the TPC-H value domain run alone through the 64-bit kernel.
@@ -76,6 +78,8 @@ struct Slot {
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t kernel_node = nullptr;  // the batched kernel's node (its grid is set per batch)
   hipKernelNodeParams kernel_params{};
+  q1::Spread spread{};           // the slot's own spread-merge tables (lone requests)
+  bool spread_dirty = false;     // a failed launch may have left partial sums in them
   std::vector<Pending> batch;
   Clock::time_point started;
   uint32_t batch_id = 0;
@@ -214,6 +218,42 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
   while (cap < n) cap <<= 1;
   q1_status st = prepare_slot(sv, s, cap, t0->layout);
   if (st != Q1_OK) return st;
+  if (n == 1 && s.spread.tables) {
+    // A lone request runs the scan kernel of q1_run with its tuned launch (the batched
+    // kernel's list of entries buys nothing for one request), merging large grids through
+    // the slot's own spread tables.
+    const q1_request& r = s.batch[0].req;
+    q1_placement p = q1::resolve_placement(sv->ctx, t0->layout, Q1_PLACE_AUTO);
+    q1_launch l;
+    q1::resolve_launch(sv->ctx, t0->layout, p, r.row_count, nullptr, &l);
+    q1::LaunchShape shape;
+    const char* why = "";
+    st = q1::make_shape(p, t0->layout, &l, r.row_begin, r.row_begin + r.row_count, &shape, &why);
+    if (st == Q1_OK && q1::scan_workspace_bytes(p, shape) == 0) {
+      q1::ScanArgs a;
+      a.cols = t0->dev;
+      a.row_begin = r.row_begin;
+      a.row_end = r.row_begin + r.row_count;
+      a.cutoff_day = t0->layout == Q1_LAYOUT_COMPRESSED_PUSHDOWN ? t0->filter_cutoff : r.cutoff_day;
+      a.out = s.d_out;
+      if (s.spread_dirty) {
+        st = q1::launch_zero(s.spread.tables, q1::kSpreadTables, s.stream);
+        if (st != Q1_OK) return st;
+        s.spread_dirty = false;
+      }
+      st = q1::launch_zero(s.d_out, 1, s.stream);
+      if (st == Q1_OK) st = q1::launch_scan(p, t0->layout, true, a, shape, nullptr, s.stream, &s.spread);
+      if (st != Q1_OK) s.spread_dirty = true;
+      if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, sizeof(q1_device_aggregates), hipMemcpyDeviceToHost,
+                                        s.stream) != hipSuccess)
+        st = Q1_EDEVICE;
+      if (st == Q1_OK && hipEventRecord(s.done, s.stream) != hipSuccess) {
+        (void)hipStreamSynchronize(s.stream);
+        st = Q1_EDEVICE;
+      }
+      return st;
+    }
+  }
   const uint32_t lane_rows = q1::batch_lane_rows(sv->ctx, t0->layout, sv->placement);
   q1::BatchDesc* d = reinterpret_cast<q1::BatchDesc*>(s.h_buf + 1);
   uint64_t entries = 0;
@@ -412,12 +452,19 @@ q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server**
     const q1_placement best = ctx->best[Q1_LAYOUT_COMPRESSED];
     sv->placement = best == Q1_PLACE_PER_THREAD_SHARED_MEM ? best : Q1_PLACE_IN_REGISTERS;
     sv->slots.resize(c.max_inflight);
+    const size_t spread_bytes = q1::kSpreadTables * sizeof(q1_device_aggregates);
     for (auto& s : sv->slots) {
       if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
-          hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+          hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+          hipMalloc(&s.spread.tables, spread_bytes) != hipSuccess ||
+          hipMemset(s.spread.tables, 0, spread_bytes) != hipSuccess) {
         q1_server_destroy(sv);
-        return q1::fail(ctx, Q1_EDEVICE, "stream creation failed");
+        return q1::fail(ctx, Q1_EDEVICE, "stream or spread-table creation failed");
       }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+      q1_server_destroy(sv);
+      return q1::fail(ctx, Q1_EDEVICE, "device synchronization failed");
     }
     for (size_t i = 0; i < sv->slots.size(); i++) sv->slots[i].waiter = std::thread(slot_waiter, sv, i);
     sv->scheduler = std::thread(scheduler_loop, sv);
@@ -559,6 +606,7 @@ void q1_server_destroy(q1_server* sv) {
   for (auto& s : sv->slots) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
     free_slot_buffers(s);
+    if (s.spread.tables) (void)hipFree(s.spread.tables);
     if (s.done) (void)hipEventDestroy(s.done);
     if (s.stream) (void)hipStreamDestroy(s.stream);
   }
