@@ -306,7 +306,7 @@ q1_status q1_tuning_set_best_placement(q1_ctx* ctx, q1_layout layout, q1_placeme
 /* --- serving (new): continuous batching over HBM-resident tables ------------------ */
 typedef struct q1_server q1_server;
 typedef struct {
-  uint32_t max_batch;          /* requests per batch launch (<= 1024) */
+  uint32_t max_batch;          /* requests per batch launch (<= 4096) */
   uint32_t max_inflight;       /* batches on the device at once (streams) */
   uint32_t queue_capacity;     /* pending requests before Q1_EBUSY */
   uint32_t batch_window_us;    /* how long the scheduler may hold a batch back to fill it while
@@ -316,6 +316,11 @@ typedef struct {
                                   table) plus 480 B of descriptor and result per request;
                                   0 = three quarters of the device's memory.  A request whose
                                   table alone does not fit gets Q1_ENOMEM at submit. */
+  uint64_t exclusive_bytes;    /* a request that scans at least this many column bytes is
+                                  large: it runs alone (the scan kernel with its tuned launch),
+                                  and large requests run one at a time, in arrival order, while
+                                  smaller ones keep batching beside them; 0 = 512 MiB (about
+                                  75 us of HBM time), UINT64_MAX = never */
 } q1_server_config;
 
 typedef struct {

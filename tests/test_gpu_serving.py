@@ -76,6 +76,66 @@ def table_bytes(t):
     return sum(q1.column_bytes(t.layout, t.rows))
 
 
+def burst(sv, reqs):
+    """Submits every request at once; returns (status, groups, stats, t_before, t_after) each,
+    the submit call bracketed by host clock readings (seconds)."""
+    import time
+    subs = []
+    for t, b, n, c in reqs:
+        t0 = time.perf_counter()
+        st, tk = sv.submit(t, b, n, c)
+        subs.append((st, tk, t0, time.perf_counter()))
+    out = []
+    for st, tk, t0, t1 in subs:
+        assert st == q1.OK
+        st, groups, s = sv.wait(tk)
+        out.append((st, groups, s, t0, t1))
+    return out
+
+
+def test_large_requests_run_alone_one_at_a_time(ctx, tables):
+    """exclusive_bytes = 1 GiB on a 200M-row table (1.875 GB of columns): its full scans are
+    large, its 100M-row halves are not.  Each large request runs in a batch of its own and
+    starts only after the large one before it has finished; the small requests that queue
+    meanwhile still batch together on the other slot."""
+    big = ctx.generate(q1.COMPRESSED, SEED + 7, 200_000_000)
+    small = tables[1]
+    try:
+        reqs = []
+        for i in range(6):
+            reqs.append((big, 0, big.rows, 729999 - i))
+            reqs.append((big, 50_000_000 * (i % 3), 100_000_000, 729980 + i))
+            reqs += [(small, 1000 * k, 500_000, 729990 + k) for k in range(12)]
+        note(f"burst of {len(reqs)} requests, exclusive_bytes 1 GiB")
+        with Server(ctx, max_batch=64, max_inflight=2, exclusive_bytes=1 << 30) as sv:
+            res = burst(sv, reqs)
+        larges, small_sizes = [], []
+        for (t, b, n, c), (st, groups, s, t0, t1) in zip(reqs, res):
+            assert st == q1.OK and groups == ctx.run(t, b, n, c)
+            if n == big.rows:
+                assert s.batch_size == 1
+                # host-clock bounds of the request's start and end (the submit call's own
+                # duration is the uncertainty)
+                larges.append((t0 + s.submit_to_start_us * 1e-6, t1 + s.submit_to_start_us * 1e-6,
+                               t0 + s.latency_us * 1e-6, t1 + s.latency_us * 1e-6, s.batch_id))
+            elif t is small:
+                small_sizes.append(s.batch_size)
+        larges.sort(key=lambda x: x[4])
+        for x, y in zip(larges, larges[1:]):
+            assert y[1] >= x[2], "a large request started before the previous one finished"
+        note(f"small-request batch sizes {sorted(set(small_sizes))}")
+        assert max(small_sizes) > 1
+        # exclusive_bytes = 2**64 - 1: no request is large, full scans batch together too
+        reqs = [(big, 0, big.rows, 729999 - i) for i in range(8)]
+        with Server(ctx, max_batch=64, max_inflight=1, exclusive_bytes=2**64 - 1) as sv:
+            res = burst(sv, reqs)
+        assert max(r[2].batch_size for r in res) > 1
+        for (t, b, n, c), (st, groups, *_) in zip(reqs, res):
+            assert st == q1.OK and groups == ctx.run(t, b, n, c)
+    finally:
+        big.free()
+
+
 def test_admission_by_memory_budget(ctx, tables):
     """In-flight work pins its tables' columns (once per table) and 480 B of descriptor and
     result per request: a budget of one table plus four requests' state keeps at most four

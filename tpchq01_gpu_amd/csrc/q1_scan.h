@@ -897,7 +897,7 @@ struct AccRegisters {
     }
   }
   // Called after at most 16 rows: moves the narrow sums into the wide ones.
-  __device__ __forceinline__ void flush() {
+  __device__ __forceinline__ void flush(bool = false) {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       if (ballot(ns[g] != 0)) {
@@ -974,7 +974,7 @@ struct AccPerThread {
       }
     }
   }
-  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void flush(bool = false) {}
   __device__ __forceinline__ void totals(LaneTotals& t, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
@@ -991,19 +991,21 @@ struct AccPerThread {
 // (src/kernels/ht_in_local_mem.cuh:54-59) lives in the lane's private (scratch) memory:
 // [group][count|discount|quantity, price, discount product, charge], 64-bit words.  Rows
 // are first summed over kLocalFlushRows rows of the lane in registers, with the
-// wavefront-uniform group skip of AccRegisters; those sums are then added into the private
-// table, one read-modify-write per word of each group the lane touched.  A row costs
-// registers only; the table sees one update per word and group every kLocalFlushRows rows.
-// (With an update every 16 rows the private tables of the resident wavefronts outgrew the
-// caches and the launch took 3.2 ms at 600M rows.)
-constexpr uint32_t kLocalFlushRows = 64;
+// wavefront-uniform group skip of AccRegisters; those sums then go into the private table:
+// the first time as plain stores (the table is never zero-filled), later as one
+// read-modify-write of the whole table, and once more after the lane's last row unless
+// the last periodic update just happened.  A row costs registers only.  The private tables
+// of the resident wavefronts (192 B per lane) outgrow the L2, so their updates reach HBM;
+// the update interval sets their cost at 600M compressed rows: every 16 rows 3.2 ms,
+// 64 rows 1.46 ms, 256 rows 1.35 ms, 1024 rows 1.19 ms.
+constexpr uint32_t kLocalFlushRows = 1024;
 struct AccLocal {
   static constexpr bool kPackedSmall = false;
   // register sums of at most kLocalFlushRows rows: count, discount and quantity in the
-  // pack_small64 layout (count < 2^7, discount < 2^10, quantity < 2^19), price below 2^30
+  // pack_small64 layout (exact up to 16384 rows), price, discount product and charge
   static constexpr uint32_t kTableWords = kNumGroups * 4;
-  uint64_t ns[kNumGroups], nd[kNumGroups], nc[kNumGroups];
-  uint32_t np[kNumGroups];
+  static constexpr uint32_t kFlushCalls = kLocalFlushRows / 16;
+  uint64_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups], nc[kNumGroups];
   uint32_t calls;
   // The table (kTableWords words of the caller's private array) is used only through this
   // pointer, which an empty asm hides from the optimizer: the array cannot become
@@ -1015,12 +1017,7 @@ struct AccLocal {
     t = reinterpret_cast<uint64_t*>(private_table);
     __asm__ volatile("" : "+v"(t));
 #pragma unroll
-    for (int g = 0; g < kNumGroups; g++) {
-      ns[g] = nd[g] = nc[g] = 0;
-      np[g] = 0;
-#pragma unroll
-      for (int f = 0; f < 4; f++) t[g * 4 + f] = 0;
-    }
+    for (int g = 0; g < kNumGroups; g++) ns[g] = np[g] = nd[g] = nc[g] = 0;
   }
   __device__ __forceinline__ void add(const FastRow& r) {
     const uint64_t s = pack_small64(r);
@@ -1040,34 +1037,43 @@ struct AccLocal {
       }
     }
   }
-  // Called every 16 rows of a lane (at the same points in every lane); every
-  // kLocalFlushRows rows the register sums go into the private table.
-  __device__ __forceinline__ void flush() {
-    if (++calls % (kLocalFlushRows / 16) != 0) return;
+  // Called every 16 rows of a lane (at the same points in every lane) and after its last
+  // row (last = true); every kLocalFlushRows rows, and at the end, the register sums go into
+  // the private table.
+  __device__ __forceinline__ void flush(bool last = false) {
+    static_assert(kLocalFlushRows <= 16384, "pack_small64 is exact up to 16384 rows");
+    if (last) {
+      if (calls > 0 && calls % kFlushCalls == 0) return;  // nothing summed since the last update
+    } else if (++calls % kFlushCalls != 0) {
+      return;
+    }
+    // All words are loaded before any is added to and stored, with no branch between the
+    // loads: the lane waits for its private memory once per update, not once per group
+    // (groups without rows add zero).
+    const bool first = calls <= kFlushCalls;
+    uint64_t w[kTableWords];
+#pragma unroll
+    for (uint32_t i = 0; i < kTableWords; i++) w[i] = first ? 0 : t[i];
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      if (ballot(ns[g] != 0)) {
-        if (ns[g] != 0) {
-          uint64_t* w = t + g * 4;
-          w[0] += ns[g];
-          w[1] += np[g];
-          w[2] += nd[g];
-          w[3] += nc[g];
-        }
-        ns[g] = nd[g] = nc[g] = 0;
-        np[g] = 0;
-      }
+      w[g * 4 + 0] += ns[g];
+      w[g * 4 + 1] += np[g];
+      w[g * 4 + 2] += nd[g];
+      w[g * 4 + 3] += nc[g];
+      ns[g] = np[g] = nd[g] = nc[g] = 0;
     }
+#pragma unroll
+    for (uint32_t i = 0; i < kTableWords; i++) t[i] = w[i];
   }
-  // The private table plus the register sums not yet added to it.
+  // The private table (after the last flush).
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
       const uint64_t* w = t + g * 4;
-      unpack_small64(w[0] + ns[g], o.v[g]);
-      o.v[g][F_PRICE] = w[1] + np[g];
-      o.v[g][F_DPROD] = w[2] + nd[g];
-      o.v[g][F_CHARGE] = w[3] + nc[g];
+      unpack_small64(w[0], o.v[g]);
+      o.v[g][F_PRICE] = w[1];
+      o.v[g][F_DPROD] = w[2];
+      o.v[g][F_CHARGE] = w[3];
     }
   }
   static constexpr uint32_t lds_bytes(uint32_t) { return 0; }
@@ -1105,7 +1111,7 @@ struct AccShared {
       __hip_atomic_fetch_add(base + 3 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
-  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void flush(bool = false) {}
   // NARROW tables hand over the packed (count | discount | quantity) word in F_CNT: at
   // most 256 rows per lane, a wavefront's sum of it still fits every field (count
   // < 2^17, discount < 2^20, quantity < 2^27), so the block merge sums one word for three.
@@ -1158,7 +1164,7 @@ struct AccSharedPacked {
       __hip_atomic_fetch_add(base + 2 * B, fast_charge(r), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   }
-  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void flush(bool = false) {}
   // The block merge gets count, discount and quantity re-packed in the pack_small64 layout
   // (a wavefront's sums of them fit it: count <= 64 * 256, discount <= 64 * 3840,
   // quantity <= 64 * 16128), so it sums one word for the three.
@@ -1202,7 +1208,7 @@ struct AccGlobal {
   }
   __device__ __forceinline__ void init(char* lds) { regs.init(lds); }
   __device__ __forceinline__ void add(const FastRow& r) { regs.add(r); }
-  __device__ __forceinline__ void flush() { regs.flush(); }
+  __device__ __forceinline__ void flush(bool = false) { regs.flush(); }
   // The block's whole merge (called by every thread of the block).
   __device__ __forceinline__ void merge(q1_device_aggregates* out, uint64_t qty_scale) const {
     LaneTotals o;
@@ -1507,7 +1513,7 @@ __device__ __forceinline__ void scan_block(const ScanArgs& a, uint64_t block, ui
       if ((n + 1) % FLUSH == 0) acc.flush();  // at most 16 rows per lane between flushes
     }
   }
-  acc.flush();
+  acc.flush(true);
 
   const uint64_t qty_scale =
       (L == Q1_LAYOUT_COMPRESSED || L == Q1_LAYOUT_COMPRESSED_PUSHDOWN) ? kQuantityScaleCompressed : 1;
@@ -1975,6 +1981,10 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
   const uint64_t total = hdr->total_blocks;
   uint64_t v = blockIdx.x, next_begin = 0;
   uint32_t r = 0;
+  // A grid of one block per entry (what the server launches for a batch with fewer entries
+  // than its tuned grid) claims nothing: block b runs entry b only, with no atomic on the
+  // shared counter.
+  const bool claims = gridDim.x < total;
   while (v < total) {
     if (v >= next_begin) r = find_request(descs, n, r, v, total, next_begin);
     const BatchDesc d = descs[r];
@@ -1985,7 +1995,7 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
     // uniform-address atomicAdd into a wavefront-aggregated one that waits for the
     // returned value at once; this value is needed only at the entry's end.
     uint32_t claim = 0;
-    if (threadIdx.x == 0) claim = atomicInc(claimed, 0xFFFFFFFFu);
+    if (claims && threadIdx.x == 0) claim = atomicInc(claimed, 0xFFFFFFFFu);
     ScanArgs a;
     a.cols = load_global_columns(&descs[r].cols);
     a.row_begin = d.row_begin;
@@ -1994,7 +2004,7 @@ batch_scan(const BatchHeader* __restrict__ hdr, const BatchDesc* __restrict__ de
     a.out = outs + r;
     a.out_mask = 0;
     scan_block<L, P, kBatchBlock>(a, v - d.block_begin, tpt, nullptr, lds, outs + r);
-    if (threadIdx.x == 0) next_entry = gridDim.x + claim;
+    if (threadIdx.x == 0) next_entry = claims ? gridDim.x + claim : (uint32_t)total;  // total < 2^31
     // Also orders this entry's reads of the merge scratch before the next entry's writes.
     __syncthreads();
     // The same value in every lane: readfirstlane keeps the entry's bounds in scalar

@@ -3,21 +3,20 @@
 //   submit(): on an idle server (nothing queued, a slot free, the request admissible) the
 //     submitting thread launches the request at once as a batch of one; otherwise it
 //     joins the pending queue (bounded: Q1_EBUSY when full).
-//   scheduler thread: whenever an in-flight slot is free, takes up to max_batch pending
-//     requests of one layout (optionally holding them back batch_window_us for more while
-//     other batches are still running), admits them while the device memory the
-//     in-flight work pins stays within memory_budget, and launches them as one batch on
-//     the slot's stream (a hipGraph per slot: descriptor copy, zero fill kernel, batched
-//     kernel, result copy).
+//   scheduler thread: whenever an in-flight slot is free, admits pending requests in
+//     arrival order (optionally holding them back batch_window_us for more while other
+//     batches are still running) and launches them as one batch on the slot's stream (a
+//     hipGraph per slot: descriptor copy, zero fill kernel, batched kernel, result copy).
 //   completion: a thread per slot waits for the slot's batch and publishes every result
 //     with its latency; a client waiting for a request of a launched batch waits for the
 //     device itself and publishes the batch first if it gets there first.
-// The batched kernel of a served batch runs one block per block-sized entry of its
-// requests (not a persistent grid), so the blocks of batches on different slots
-// interleave on the device: a small request that arrives while a large one runs starts at
-// once on a free slot and does not wait for the large one to drain.  A batch of one
-// request runs q1_run's scan kernel with its tuned launch instead, merging through the
-// slot's own spread tables.
+// Large requests (at least exclusive_bytes of columns to scan) run alone, through q1_run's
+// scan kernel with its tuned launch and the slot's own spread tables, and one at a time in
+// arrival order: two large scans sharing the device would both finish late, which is what
+// sets the latency tail.  Smaller requests batch beside them on the other slots, so a short
+// request that arrives while a large one runs starts at once instead of queueing behind it.
+// A batch runs the persistent batched kernel with its tuned grid, or one block per entry
+// when the batch has fewer entries than that grid.
 // Admission by device memory: a request pins its table's columns (counted once per table
 // while any of its requests is in flight) and its descriptor and result table.
 // Requests on tables outside the TPC-H value domain run alone through the 64-bit kernel.
@@ -78,6 +77,7 @@ struct Slot {
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t kernel_node = nullptr;  // the batched kernel's node (its grid is set per batch)
   hipKernelNodeParams kernel_params{};
+  uint32_t grid = 0;             // the batched kernel's tuned (persistent) grid
   q1::Spread spread{};           // the slot's own spread-merge tables (lone requests)
   bool spread_dirty = false;     // a failed launch may have left partial sums in them
   std::vector<Pending> batch;
@@ -102,6 +102,7 @@ struct q1_server {
   std::unordered_map<const q1_table*, uint32_t> pinned;    // in-flight requests per table
   uint64_t next_ticket = 1;
   uint64_t bytes_in_flight = 0;
+  uint32_t large_in_flight = 0;                            // large requests on the device
   uint32_t batches = 0;
   bool stop = false;
   std::vector<Slot> slots;
@@ -177,6 +178,7 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
       }
     }
   }
+  s.grid = blocks;
   s.capacity = cap;
   s.layout = layout;
   return Q1_OK;
@@ -264,9 +266,10 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
   s.h_buf->tiles_per_thread = lane_rows / q1::layout_tile_rows(t0->layout);
   s.h_buf->claimed = 0;
   if (s.kernel_node) {
-    // one block per entry: block b runs entry b, its claim then lies past the list
+    // The tuned persistent grid, or one block per entry for a batch with fewer entries
+    // (block b then runs entry b and claims nothing).
     hipKernelNodeParams kp = s.kernel_params;
-    kp.gridDim = dim3((uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, 1), (1u << 24) - 1));
+    kp.gridDim = dim3((uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, 1), s.grid));
     if (hipGraphExecKernelNodeSetParams(s.exec, s.kernel_node, &kp) != hipSuccess) s.kernel_node = nullptr;
   }
   if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return Q1_EDEVICE;
@@ -284,9 +287,25 @@ uint64_t admission_cost(const q1_server* sv, const q1_request& r) {
   return request_state_bytes() + (sv->pinned.count(r.table) ? 0 : table_bytes(r.table));
 }
 
+// Whether `r` scans at least exclusive_bytes of columns (its share of the table's bytes).
+bool is_large(const q1_server* sv, const q1_request& r) {
+  const q1_table* t = r.table;
+  if (!t->rows) return false;
+  const unsigned __int128 scanned = (unsigned __int128)table_bytes(t) * r.row_count / t->rows;
+  return scanned >= sv->cfg.exclusive_bytes;
+}
+
+// Whether `r` may go on the device now: it fits the memory budget and, if large, no other
+// large request is in flight (server lock held).
+bool admissible(const q1_server* sv, const q1_request& r) {
+  if (sv->bytes_in_flight + admission_cost(sv, r) > sv->cfg.memory_budget) return false;
+  return sv->large_in_flight == 0 || !is_large(sv, r);
+}
+
 // Puts a pending request into a slot's batch (server lock held).
 void take(q1_server* sv, size_t slot, const Pending& p) {
   sv->bytes_in_flight += admission_cost(sv, p.req);
+  if (is_large(sv, p.req)) sv->large_in_flight++;
   sv->pinned[p.req.table]++;
   sv->inflight[p.ticket] = Where{slot, sv->slots[slot].batch_id};
   sv->slots[slot].batch.push_back(p);
@@ -309,6 +328,7 @@ void publish(q1_server* sv, Slot& s, q1_status st, uint32_t id) {
     sv->finished[s.batch[i].ticket] = f;
     sv->inflight.erase(s.batch[i].ticket);
     const q1_table* t = s.batch[i].req.table;
+    if (is_large(sv, s.batch[i].req)) sv->large_in_flight--;
     sv->bytes_in_flight -= request_state_bytes();
     if (--sv->pinned[t] == 0) {
       sv->pinned.erase(t);
@@ -390,29 +410,42 @@ void scheduler_loop(q1_server* sv) {
     if (idx == sv->slots.size() || sv->queue.empty()) continue;
     Slot* slot = &sv->slots[idx];
     slot->batch_id = ++sv->batches;
-    // Admission: requests of the front request's layout, within the memory budget.
-    const q1_table* t0 = sv->queue.front().req.table;
-    slot->single = !t0->fast;
-    const uint32_t limit = slot->single ? 1 : sv->cfg.max_batch;
+    // Admission in arrival order: the first admissible request opens the batch.  A large
+    // request (or one outside the TPC-H domain) runs alone; smaller ones of the same
+    // layout join it up to max_batch.  A request beyond the memory budget ends the scan, so
+    // later ones cannot starve it; a large request while another is in flight, or one of
+    // another layout, keeps its place in the queue.
     std::deque<Pending> rest;
-    while (!sv->queue.empty() && slot->batch.size() < limit) {
+    const q1_table* t0 = nullptr;
+    bool alone = false;
+    while (!sv->queue.empty()) {
       Pending p = sv->queue.front();
       sv->queue.pop_front();
-      const bool same = p.req.table->layout == t0->layout && p.req.table->fast == t0->fast;
-      const bool fits = sv->bytes_in_flight + admission_cost(sv, p.req) <= sv->cfg.memory_budget;
-      if (!same || !fits) {
+      if (sv->bytes_in_flight + admission_cost(sv, p.req) > sv->cfg.memory_budget) {
         rest.push_back(p);
-        if (!fits) break;
+        break;
+      }
+      const bool large = is_large(sv, p.req);
+      const bool joins = t0 ? !large && p.req.table->fast && p.req.table->layout == t0->layout
+                            : !large || sv->large_in_flight == 0;
+      if (!joins) {
+        rest.push_back(p);
         continue;
       }
+      if (!t0) {
+        t0 = p.req.table;
+        alone = large || !t0->fast;
+        slot->single = !t0->fast;
+      }
       take(sv, idx, p);
+      if (alone || slot->batch.size() >= sv->cfg.max_batch) break;
     }
     while (!rest.empty()) {
       sv->queue.push_front(rest.back());
       rest.pop_back();
     }
     if (slot->batch.empty()) {
-      // nothing admissible until in-flight memory frees
+      // nothing admissible until in-flight memory frees or the large request in flight ends
       sv->cv_sched.wait(lk);
       continue;
     }
@@ -429,13 +462,14 @@ extern "C" {
 q1_status q1_server_create(q1_ctx* ctx, const q1_server_config* cfg, q1_server** out) {
   if (!ctx || !out) return Q1_EINVAL;
   *out = nullptr;
-  q1_server_config c{256, 2, 65536, 0, 0};
+  q1_server_config c{256, 2, 65536, 0, 0, 512ull << 20};
   if (cfg) {
     if (cfg->max_batch) c.max_batch = cfg->max_batch;
     if (cfg->max_inflight) c.max_inflight = cfg->max_inflight;
     if (cfg->queue_capacity) c.queue_capacity = cfg->queue_capacity;
     c.batch_window_us = cfg->batch_window_us;
     c.memory_budget = cfg->memory_budget;
+    if (cfg->exclusive_bytes) c.exclusive_bytes = cfg->exclusive_bytes;
   }
   if (c.max_batch > 4096 || c.max_inflight > 8) return q1::fail(ctx, Q1_EINVAL, "max_batch <= 4096, max_inflight <= 8");
   try {
@@ -492,7 +526,7 @@ q1_status q1_server_submit(q1_server* sv, const q1_request* req, uint64_t* ticke
   if (sv->queue.empty())
     for (size_t i = 0; i < sv->slots.size(); i++)
       if (!sv->slots[i].busy) idx = i;
-  if (idx < sv->slots.size() && sv->bytes_in_flight + admission_cost(sv, *req) <= sv->cfg.memory_budget) {
+  if (idx < sv->slots.size() && admissible(sv, *req)) {
     Slot& slot = sv->slots[idx];
     slot.batch_id = ++sv->batches;
     slot.single = !t->fast;

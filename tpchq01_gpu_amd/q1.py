@@ -116,7 +116,7 @@ class Request(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("max_inflight", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("batch_window_us", C.c_uint32),
-                ("memory_budget", C.c_uint64)]
+                ("memory_budget", C.c_uint64), ("exclusive_bytes", C.c_uint64)]
 
 
 class RequestStats(C.Structure):

@@ -14,14 +14,16 @@ from . import q1
 
 class Server:
     def __init__(self, ctx: q1.Context, max_batch=256, max_inflight=2, queue_capacity=65536,
-                 batch_window_us=0, memory_budget=0):
+                 batch_window_us=0, memory_budget=0, exclusive_bytes=0):
         """memory_budget: device bytes the in-flight work may pin (the columns of each table
         with requests in flight, plus descriptor and result state per request); 0 = three
-        quarters of the device's memory."""
+        quarters of the device's memory.  exclusive_bytes: requests that scan at least this
+        many column bytes run alone and one at a time; 0 = 512 MiB, 2**64 - 1 = never."""
         if not hasattr(q1.lib, "q1_server_create"):
             raise RuntimeError("libq1hip.so was built without the server")
         self.ctx = ctx
-        cfg = q1.ServerConfig(max_batch, max_inflight, queue_capacity, batch_window_us, memory_budget)
+        cfg = q1.ServerConfig(max_batch, max_inflight, queue_capacity, batch_window_us, memory_budget,
+                              exclusive_bytes)
         h = C.c_void_p()
         ctx.check(q1.lib.q1_server_create(ctx.handle, C.byref(cfg), C.byref(h)), "q1_server_create")
         self.handle = h.value
