@@ -150,12 +150,13 @@ def test_result_merge_is_exact():
 
 def test_shipped_tuning_table_is_well_formed():
     """The autotuning table that q1_init loads next to the library: every launch shape is
-    one the kernels accept and every layout names a best placement."""
+    one the kernels accept, every layout names a best placement, and the batched kernel's
+    shapes (compressed layout, both batch placements) are ones q1_init accepts."""
     import os
 
     from tpchq01_gpu_amd import PACKAGE_DIR
     path = os.path.join(PACKAGE_DIR, "tuning", "gfx950.txt")
-    launches, best = {}, {}
+    launches, best, batch = {}, {}, {}
     for line in open(path):
         f = line.split()
         if not f or f[0].startswith("#"):
@@ -164,6 +165,11 @@ def test_shipped_tuning_table_is_well_formed():
             l, p, tpb, tpt = map(int, f[1:5])
             assert tpb in (64, 128, 256) and tpt % 16 == 0 and 16 <= tpt <= 16384, line
             launches[(l, p)] = (tpb, tpt)
+        elif f[0] == "batch":
+            l, p, per_cu, lane_rows = map(int, f[1:5])
+            most = 256 if p == q1.PER_THREAD_SHARED_MEM else 16384
+            assert 1 <= per_cu <= 64 and lane_rows % 16 == 0 and 16 <= lane_rows <= most, line
+            batch[(l, p)] = (per_cu, lane_rows)
         else:
             assert f[0] == "best", line
             best[int(f[1])] = int(f[2])
@@ -171,6 +177,7 @@ def test_shipped_tuning_table_is_well_formed():
     assert set(best) == set(layouts)
     for l in layouts:
         assert (l, best[l]) in launches
+    assert set(batch) == {(q1.COMPRESSED, q1.IN_REGISTERS), (q1.COMPRESSED, q1.PER_THREAD_SHARED_MEM)}
 
 
 def _compressed_cols(ship, qty, price, disc, tax, rf, ls):
