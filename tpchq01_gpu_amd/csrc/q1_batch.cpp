@@ -69,7 +69,7 @@ uint64_t batch_describe(BatchDesc* d, const q1_request& r, uint64_t first, uint3
   d->pad = 0;
   if (r.row_count == 0) return 0;
   const uint64_t trows = layout_tile_rows(t->layout);
-  const uint64_t tiles = (d->row_end + trows - 1) / trows - d->row_begin / trows;
+  const uint64_t tiles = (d->row_end + trows - 1) / trows - align_rows_down(d->row_begin) / trows;
   const uint64_t per_block = (uint64_t)kBatchThreads * (lane_rows / trows);
   return (tiles + per_block - 1) / per_block;
 }

@@ -16,6 +16,15 @@ struct Domain {
 
 bool domain_is_tpch(q1_layout layout, const Domain& d);
 
+// The blocks of a launch over rows [row_begin, row_end) start at row_begin rounded down to
+// a multiple of kRowAlign rows (rows before row_begin are masked off in the first block).
+// 128 rows are a whole number of tiles of every layout and kernel, and of 128-byte lines of
+// the 1-, 2- and 4-byte columns, so every wavefront load of a column starts on a line,
+// wherever the range starts: a range starting at an arbitrary row otherwise made every
+// load touch one line more and ran 10 % slower.
+constexpr uint64_t kRowAlign = 128;
+inline uint64_t align_rows_down(uint64_t row) { return row / kRowAlign * kRowAlign; }
+
 struct LaunchShape {
   uint32_t blocks, threads, tiles_per_thread;
 };

@@ -236,7 +236,7 @@ q1_status make_shape(q1_placement placement, q1_layout layout, const q1_launch* 
     return Q1_EINVAL;
   }
   const uint32_t tiles_per_thread = tpt / trows;
-  const uint64_t first = row_begin / trows, end = (row_end + trows - 1) / trows;
+  const uint64_t first = align_rows_down(row_begin) / trows, end = (row_end + trows - 1) / trows;
   const uint64_t tiles = end > first ? end - first : 0;
   const uint64_t per_block = (uint64_t)threads * tiles_per_thread;
   uint64_t blocks = (tiles + per_block - 1) / per_block;

@@ -1442,9 +1442,11 @@ __device__ __forceinline__ void scan_block(const ScanArgs& a, uint64_t block, ui
 
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
   // Block b covers kernel tiles [first_tile + b*B*tpt, +B*tpt): SUB times the host's
-  // tiles per lane, so the grid covers every row of the range.
+  // tiles per lane, so the grid covers every row of the range.  The first tile is that of
+  // row_begin rounded down to kRowAlign rows, as the host counts them.
+  static_assert(kRowAlign % TILE == 0, "kRowAlign is a whole number of tiles");
   const uint32_t tiles_per_thread = host_tiles_per_thread * SUB;
-  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t first_tile = a.row_begin / kRowAlign * kRowAlign / TILE;
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
   const uint64_t base = first_tile + block * B * tiles_per_thread;
 
@@ -1642,7 +1644,7 @@ scan_any(ScanArgs a, uint32_t host_tiles_per_thread) {
 
   const uint32_t tiles_per_thread = host_tiles_per_thread * SUB;
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
-  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t first_tile = a.row_begin / kRowAlign * kRowAlign / TILE;  // as the host counts them
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
   const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
 
@@ -1797,7 +1799,7 @@ scan_wide(ScanArgs a, uint32_t tiles_per_thread) {
   extern __shared__ uint64_t lds[];
 
   const int32_t cut = column_cutoff<L>(a.cutoff_day);
-  const uint64_t first_tile = a.row_begin / TILE;
+  const uint64_t first_tile = a.row_begin / kRowAlign * kRowAlign / TILE;  // as the host counts them
   const uint64_t end_tile = (a.row_end + TILE - 1) / TILE;
   const uint64_t base = first_tile + (uint64_t)blockIdx.x * B * tiles_per_thread;
 
