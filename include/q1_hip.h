@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define Q1_ABI_VERSION 1
+#define Q1_ABI_VERSION 2  /* 2: q1_server_config gained exclusive_bytes */
 #define Q1_NUM_GROUPS 6 /* slot = return_flag_code << 1 | line_status_code (src/data_types.h:52-80) */
 
 typedef enum {

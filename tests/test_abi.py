@@ -26,7 +26,7 @@ def test_every_declared_function_is_exported():
 
 
 def test_abi_version_and_strings():
-    assert q1.lib.q1_abi_version() == 1
+    assert q1.lib.q1_abi_version() == 2
     for s in range(8):
         assert q1.lib.q1_strerror(s)
     assert q1.lib.q1_strerror(99) == b"unknown status"
