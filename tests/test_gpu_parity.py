@@ -76,6 +76,27 @@ def test_row_ranges(ctx, oracle, layout):
             assert as_dicts(ctx.run(t, begin, end - begin, cutoff, p)) == want
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_range_starts_around_the_128_row_alignment(ctx, oracle, layout):
+    """A launch's blocks start at its first row rounded down to 128 rows and mask the rows
+    before it: starts on each side of 128-row boundaries, ranges that end inside the first
+    aligned stretch, and launch shapes whose blocks hold fewer rows than that stretch."""
+    rows = 70_000
+    ref = host_tables(rows)[q1.REFERENCE]
+    cutoff = 729999
+    t = device_table(ctx, layout, rows, cutoff)
+    ranges = [(1, rows - 1), (64, rows - 64), (127, 129), (128, 256), (129, 300), (255, 1), (100, 10), (5, 50),
+              (256 * 100 + 1, 40_000), (rows - 129, 128), (rows - 1, 1)]
+    for begin, n in ranges:
+        want, _ = oracle.naive(ref, begin, begin + n, cutoff)
+        for p in PLACEMENTS:
+            for shape in (None, q1.make_launch(64, 16)):
+                note(f"case layout={q1.LAYOUT_NAMES[layout]} range=[{begin},{begin + n}) placement={p} "
+                     f"shape={'tuned' if shape is None else '64x16'}")
+                assert as_dicts(ctx.run(t, begin, n, cutoff, p, shape)) == want
+    t.free()
+
+
 @pytest.mark.parametrize("shape", [(64, 16), (128, 256), (128, 48), (256, 4096), (256, 16384), (64, 1024)])
 def test_launch_shapes(ctx, oracle, shape):
     rows = 300_001
