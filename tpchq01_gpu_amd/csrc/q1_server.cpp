@@ -414,11 +414,14 @@ void scheduler_loop(q1_server* sv) {
     // request (or one outside the TPC-H domain) runs alone; smaller ones of the same
     // layout join it up to max_batch.  A request beyond the memory budget ends the scan, so
     // later ones cannot starve it; a large request while another is in flight, or one of
-    // another layout, keeps its place in the queue.
+    // another layout, keeps its place in the queue.  One pass looks at no more than
+    // kAdmissionScan queued requests, so a long queue of waiting large requests costs each
+    // scheduling pass a bounded amount of work.
+    constexpr size_t kAdmissionScan = 4096;
     std::deque<Pending> rest;
     const q1_table* t0 = nullptr;
     bool alone = false;
-    while (!sv->queue.empty()) {
+    for (size_t looked = 0; !sv->queue.empty() && looked < kAdmissionScan; looked++) {
       Pending p = sv->queue.front();
       sv->queue.pop_front();
       if (sv->bytes_in_flight + admission_cost(sv, p.req) > sv->cfg.memory_budget) {
