@@ -481,7 +481,7 @@ def main():
         # the autotuning table's best placement for this layout
         ctx.run(table, 0, min(rows, 1 << 20), 729999, q1.AUTO)
         placement = ctx.last_timing().placement
-    tuned = ctx.tuning_get(layout, placement)
+    tuned = ctx.tuning_resolve(layout, placement, rows)
     launch = q1.make_launch(args.threads_per_block or tuned.threads_per_block,
                             args.tuples_per_thread or tuned.tuples_per_thread, 0,
                             q1.LAUNCH_ASSUME_TPCH_DOMAIN)

@@ -297,10 +297,23 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
                            int32_t cutoff_day, q1_placement placement, const q1_launch* launch,
                            uint64_t chunk_rows, int num_streams, q1_result* out);
 
-/* --- autotuning table (new; replaces scripts/try_grid_params.py's optimum table) ---- */
+/* --- autotuning table (new; replaces scripts/try_grid_params.py's optimum table) ----
+ * q1_init reads tuning/gfx950.txt (or $Q1_TUNING_FILE): one shape per layout and
+ * placement ("launch" lines) and, optionally, shapes tuned at several row counts
+ * ("launch_at <rows> ..." lines).  A launch that leaves threads_per_block /
+ * tuples_per_thread 0 takes the shape tuned at the row count nearest its own (by ratio),
+ * halving tuples_per_thread for ranges too small to give 4 blocks per compute unit.
+ * These are configuration calls: not to be made while other calls on the context, or a
+ * server over it, are running. */
+/* Sets the shape of every row count for (layout, placement). */
 q1_status q1_tuning_set(q1_ctx* ctx, q1_layout layout, q1_placement placement,
                         const q1_launch* best);
+/* The "launch" line's shape (the largest row count swept). */
 q1_status q1_tuning_get(q1_ctx* ctx, q1_layout layout, q1_placement placement, q1_launch* out);
+/* The shape a q1_run of `rows` rows with no caller shape takes (tables inside the
+ * TPC-H value domain). */
+q1_status q1_tuning_resolve(q1_ctx* ctx, q1_layout layout, q1_placement placement, uint64_t rows,
+                            q1_launch* out);
 q1_status q1_tuning_set_best_placement(q1_ctx* ctx, q1_layout layout, q1_placement placement);
 
 /* --- serving (new): continuous batching over HBM-resident tables ------------------ */

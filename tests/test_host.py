@@ -156,7 +156,7 @@ def test_shipped_tuning_table_is_well_formed():
 
     from tpchq01_gpu_amd import PACKAGE_DIR
     path = os.path.join(PACKAGE_DIR, "tuning", "gfx950.txt")
-    launches, best, batch = {}, {}, {}
+    launches, best, batch, at = {}, {}, {}, {}
     for line in open(path):
         f = line.split()
         if not f or f[0].startswith("#"):
@@ -165,6 +165,10 @@ def test_shipped_tuning_table_is_well_formed():
             l, p, tpb, tpt = map(int, f[1:5])
             assert tpb in (64, 128, 256) and tpt % 16 == 0 and 16 <= tpt <= 16384, line
             launches[(l, p)] = (tpb, tpt)
+        elif f[0] == "launch_at":
+            n, l, p, tpb, tpt = map(int, f[1:6])
+            assert n > 0 and tpb in (64, 128, 256) and tpt % 16 == 0 and 16 <= tpt <= 16384, line
+            at.setdefault(n, {})[(l, p)] = (tpb, tpt)
         elif f[0] == "batch":
             l, p, per_cu, lane_rows = map(int, f[1:5])
             most = 256 if p == q1.PER_THREAD_SHARED_MEM else 16384
@@ -178,6 +182,12 @@ def test_shipped_tuning_table_is_well_formed():
     for l in layouts:
         assert (l, best[l]) in launches
     assert set(batch) == {(q1.COMPRESSED, q1.IN_REGISTERS), (q1.COMPRESSED, q1.PER_THREAD_SHARED_MEM)}
+    # shapes at several row counts: each count covers every tuned (layout, placement), and the
+    # largest count's shapes are the "launch" lines
+    if at:
+        for n, shapes in at.items():
+            assert set(shapes) == set(launches), n
+        assert at[max(at)] == launches
 
 
 def _compressed_cols(ship, qty, price, disc, tax, rf, ls):

@@ -2,11 +2,13 @@
 scripts/try_grid_params.py and the hand-entered optimum table of
 scripts/benchmark_with_optimal_params.py:39-55).
 
-Scan shapes: for every (layout, placement) it sweeps threads_per_block x tuples_per_thread,
-times the kernel alone with HIP events (q1_last_timing().kernel_ms, the minimum over
-repetitions), skips shapes that leave fewer than 2 blocks per compute unit (as
-try_grid_params.py:78-80 does), and rotates between table copies when the table is
-smaller than the 256 MB Infinity Cache so the timing reflects HBM.
+Scan shapes: for every row count asked (by default 600M, 60M and 6M rows) and every
+(layout, placement) it sweeps threads_per_block x tuples_per_thread, times the kernel alone
+with HIP events (q1_last_timing().kernel_ms, the minimum over repetitions), skips shapes
+that leave fewer than 2 blocks per compute unit (as try_grid_params.py:78-80 does), and
+rotates between table copies when the table is smaller than the 256 MB Infinity Cache so
+the timing reflects HBM.  The best shape differs with the row count: shorter launches
+favour fewer rows per lane, whose last round of blocks leaves less of the chip idle.
 
 Batch shapes: for each batch placement (the register table and the LDS table) it sweeps
 the batched kernel's grid (blocks per compute unit) x rows per lane on the 256-request
@@ -14,15 +16,18 @@ batch workload (uneven log-uniform requests, their own cutoffs), timing q1_run_b
 kernel.
 
 It writes
-  tuning/<arch>.txt    "launch <layout> <placement> <tpb> <tpt>", "best <layout> <placement>"
-                       and "batch <layout> <placement> <blocks per CU> <rows per lane>"
+  tuning/<arch>.txt    "launch <layout> <placement> <tpb> <tpt>" and "best <layout> <placement>"
+                       lines (the largest row count swept), "launch_at <rows> <layout>
+                       <placement> <tpb> <tpt>" lines (every row count, when several were
+                       swept; a launch takes the one nearest its own row count) and
+                       "batch <layout> <placement> <blocks per CU> <rows per lane>"
                        lines, read by q1_init (or Q1_TUNING_FILE)
   tuning/<arch>.json   every measurement
 and `--report` prints the optimum per layout and placement as a table
 (tuning/<arch>_report.md), the place of the reference's per-configuration optimum list and
 its plots (scripts/benchmark_with_optimal_params.py:39-55, scripts/plots.R).
 
-Usage: python -m tpchq01_gpu_amd.autotune [--rows N] [--layouts compressed,...] [--batch]
+Usage: python -m tpchq01_gpu_amd.autotune [--rows N[,N...]] [--layouts compressed,...] [--batch]
        [--merge] [--report]
 """
 from __future__ import annotations
@@ -37,7 +42,8 @@ import time
 from . import PACKAGE_DIR, q1
 
 THREADS = [64, 128, 256]
-TUPLES = [128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+TUPLES = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+ROWS = [600_000_000, 60_000_000, 6_000_000]
 BATCH_BLOCKS_PER_CU = [1, 2, 3, 4, 6, 8]
 BATCH_LANE_ROWS = {q1.IN_REGISTERS: [256, 512, 1024, 2048], q1.PER_THREAD_SHARED_MEM: [64, 128, 192, 256]}
 MALL_BYTES = 256 << 20
@@ -116,9 +122,20 @@ def sweep_batch(ctx: q1.Context, layout: int, spec, placements, reps: int, log=p
     return results
 
 
-def best_table(results):
+def sizes_of(results):
+    """The row counts measured, largest first."""
+    return sorted({r["rows"] for r in results}, reverse=True)
+
+
+def best_table(results, rows=None):
+    """The fastest shape per (layout, placement) at `rows` rows (default: the largest row
+    count measured), and the fastest placement per layout among those."""
+    if rows is None and results:
+        rows = sizes_of(results)[0]
     best = {}
     for r in results:
+        if r["rows"] != rows:
+            continue
         k = (r["layout"], r["placement"])
         if k not in best or r["kernel_ms"] < best[k]["kernel_ms"]:
             best[k] = r
@@ -130,24 +147,35 @@ def best_table(results):
 
 
 def merge_results(old, new):
-    """Measurements of `old` whose (layout, placement) `new` does not sweep again, plus `new`."""
-    swept = {(r["layout"], r["placement"]) for r in new}
-    return [r for r in old if (r["layout"], r["placement"]) not in swept] + list(new)
+    """Measurements of `old` whose (layout, placement, rows) `new` does not sweep again, plus
+    `new`."""
+    swept = {(r["layout"], r["placement"], r["rows"]) for r in new}
+    return [r for r in old if (r["layout"], r["placement"], r["rows"]) not in swept] + list(new)
 
 
-def write_tables(results, out_dir: str, device: str, rows: int, batch_results=(), arch: str = "gfx950"):
-    """Writes tuning/<arch>.txt (read by q1_init) and tuning/<arch>.json."""
+def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arch: str = "gfx950"):
+    """Writes tuning/<arch>.txt (read by q1_init) and tuning/<arch>.json.  `rows` (one row
+    count or a list) is recorded as the sweep's; the lines come from the measurements."""
     os.makedirs(out_dir, exist_ok=True)
+    sizes = sizes_of(results)
     best, per_layout = best_table(results)
     bbest, _ = best_table(batch_results)
     with open(os.path.join(out_dir, f"{arch}.txt"), "w") as f:
-        f.write(f"# written by tpchq01_gpu_amd.autotune on {device}, every layout swept at {rows} "
+        swept = ", ".join(str(n) for n in sizes)
+        f.write(f"# written by tpchq01_gpu_amd.autotune on {device}, every layout swept at {swept} "
                 f"device-resident rows:\n")
         f.write("#   launch <layout> <placement> <threads_per_block> <tuples_per_thread>   /   best <layout> <placement>\n")
+        if len(sizes) > 1:
+            f.write("#   launch_at <rows> <layout> <placement> <threads_per_block> <tuples_per_thread>"
+                    "   (a launch takes the row count nearest its own)\n")
         f.write("#   batch <layout> <placement> <blocks_per_cu> <tuples_per_thread>   (the batched kernel, 256 threads)\n")
         f.write("# (layouts 0 reference, 1 uncompressed, 2 compressed, 3 compressed + pushdown bitmap)\n")
         for (l, p), r in sorted(best.items()):
             f.write(f"launch {l} {p} {r['tpb']} {r['tpt']}\n")
+        if len(sizes) > 1:
+            for n in sorted(sizes):
+                for (l, p), r in sorted(best_table(results, n)[0].items()):
+                    f.write(f"launch_at {n} {l} {p} {r['tpb']} {r['tpt']}\n")
         for l, r in sorted(per_layout.items()):
             f.write(f"best {l} {r['placement']}\n")
         for (l, p), r in sorted(bbest.items()):
@@ -156,24 +184,28 @@ def write_tables(results, out_dir: str, device: str, rows: int, batch_results=()
         json.dump({"device": device, "rows": rows, "measurements": list(results),
                    "batch_measurements": list(batch_results),
                    "best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
-                            for (l, p), r in sorted(best.items())],
+                            for n in sizes for (l, p), r in sorted(best_table(results, n)[0].items())],
                    "batch_best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
                                   for (l, p), r in sorted(bbest.items())]}, f, indent=1)
     return best, per_layout
 
 
 def report(table: dict) -> str:
-    """Markdown: the optimum launch per layout and placement, with its kernel time, rate and
-    fraction of HBM peak, a bar per placement, and the batched kernel's optimum."""
-    best, per_layout = best_table(table["measurements"])
-    lines = [f"# Tuned launch shapes on {table.get('device', '?')} ({table.get('rows', '?')} rows per layout)", "",
-             "| layout | placement | threads/block | tuples/thread | kernel ms | GB/s | % of 8 TB/s | |",
-             "|---|---|---|---|---|---|---|---|"]
-    for (l, p), r in sorted(best.items()):
-        frac = r["gbps"] / HBM_PEAK_GBS
-        mark = " **best**" if per_layout[l] is r else ""
-        lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]}{mark} | {r['tpb']} | {r['tpt']} | "
-                     f"{r['kernel_ms']:.3f} | {r['gbps']:.0f} | {100 * frac:.1f} | `{'#' * round(40 * frac)}` |")
+    """Markdown: per row count swept, the optimum launch per layout and placement, with its
+    kernel time, rate and fraction of HBM peak and a bar per placement; then the batched
+    kernel's optimum."""
+    sizes = sizes_of(table["measurements"])
+    lines = [f"# Tuned launch shapes on {table.get('device', '?')}"]
+    for n in sizes:
+        best, per_layout = best_table(table["measurements"], n)
+        lines += ["", f"{n:,} rows per layout:", "",
+                  "| layout | placement | threads/block | tuples/thread | kernel ms | GB/s | % of 8 TB/s | |",
+                  "|---|---|---|---|---|---|---|---|"]
+        for (l, p), r in sorted(best.items()):
+            frac = r["gbps"] / HBM_PEAK_GBS
+            mark = " **best**" if per_layout[l] is r else ""
+            lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]}{mark} | {r['tpb']} | {r['tpt']} | "
+                         f"{r['kernel_ms']:.3f} | {r['gbps']:.0f} | {100 * frac:.1f} | `{'#' * round(40 * frac)}` |")
     bbest, _ = best_table(table.get("batch_measurements", []))
     if bbest:
         lines += ["", "Batched kernel (256 threads per block):", "",
@@ -188,7 +220,8 @@ def report(table: dict) -> str:
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--rows", type=int, default=600_000_000)
+    ap.add_argument("--rows", default=",".join(str(n) for n in ROWS),
+                    help="row counts to sweep the scan shapes at (comma-separated)")
     ap.add_argument("--layouts", default="compressed", help="layouts whose scan shapes are swept ('' = none)")
     ap.add_argument("--placements", default="in_registers,in_registers_per_thread,local_mem,shared_mem_per_thread,global")
     ap.add_argument("--reps", type=int, default=3)
@@ -219,8 +252,10 @@ def main(argv=None):
         info = ctx.device_info()
         device = f"MI355X ({info.arch.decode().split(':')[0]})"
         results = []
-        for l in layouts:
-            results += sweep(ctx, l, args.rows, placements, args.reps, log=out)
+        rows = [int(x) for x in args.rows.split(",") if x]
+        for n in rows:
+            for l in layouts:
+                results += sweep(ctx, l, n, placements, args.reps, log=out)
         batch_results = []
         if args.batch:
             from . import workloads
@@ -228,7 +263,7 @@ def main(argv=None):
                                         [q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM], args.reps, log=out)
     results = merge_results(old.get("measurements", []), results)
     batch_results = merge_results(old.get("batch_measurements", []), batch_results)
-    best, per_layout = write_tables(results, args.out, device, args.rows, batch_results)
+    best, per_layout = write_tables(results, args.out, device, rows if len(rows) > 1 else rows[0], batch_results)
     for l, r in per_layout.items():
         print(f"best {q1.LAYOUT_NAMES[l]}: {q1.PLACEMENT_NAMES[r['placement']]} tpb={r['tpb']} tpt={r['tpt']} "
               f"{r['kernel_ms']:.3f} ms {r['gbps']:.1f} GB/s", flush=True)

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -46,9 +47,29 @@ q1_placement resolve_placement(q1_ctx* ctx, q1_layout layout, q1_placement p) {
   return p == Q1_PLACE_AUTO ? ctx->best[layout] : p;
 }
 
+// The tuned shape for a range of `rows` rows: of the shapes tuned at several row counts,
+// the one whose row count is nearest by ratio (the larger on a tie); the single tuned
+// shape when the table has none.
+static q1_launch tuned_for_rows(const q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows) {
+  q1_launch t = ctx->tuning[layout][p];
+  const std::vector<SizedLaunch>& at = ctx->tuning_at[layout][p];
+  if (at.empty()) return t;
+  const double r = (double)std::max<uint64_t>(rows, 1);
+  double best = 0;
+  for (size_t i = 0; i < at.size(); i++) {
+    const double d = std::fabs(std::log(r / (double)at[i].rows));
+    if (i == 0 || d <= best) {
+      best = d;
+      t.threads_per_block = at[i].threads_per_block;
+      t.tuples_per_thread = at[i].tuples_per_thread;
+    }
+  }
+  return t;
+}
+
 void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows, const q1_launch* in,
                     q1_launch* out, bool any_value_kernel) {
-  const q1_launch& t = ctx->tuning[layout][p];
+  const q1_launch t = tuned_for_rows(ctx, layout, p, rows);
   const bool any_value = any_value_kernel && !(in && (in->flags & Q1_LAUNCH_ASSUME_TPCH_DOMAIN));
   const uint32_t tuned_threads = any_value ? 256 : t.threads_per_block;
   const uint32_t tuned_tuples = any_value ? 1024 : t.tuples_per_thread;
@@ -56,9 +77,9 @@ void resolve_launch(q1_ctx* ctx, q1_layout layout, q1_placement p, uint64_t rows
   out->tuples_per_thread = in && in->tuples_per_thread ? in->tuples_per_thread : tuned_tuples;
   out->num_blocks = in ? in->num_blocks : 0;
   out->flags = in ? in->flags : 0;
-  // The tuned tuples_per_thread is measured on large tables; a smaller range halves it
-  // until the grid has at least 4 blocks per compute unit, so small requests still
-  // spread over the whole chip.  A caller-chosen value is kept as given.
+  // A range smaller than the row counts the shape was tuned at halves its tuples_per_thread
+  // until the grid has at least 4 blocks per compute unit, so small requests still spread
+  // over the whole chip.  A caller-chosen value is kept as given.
   if (!(in && in->tuples_per_thread) && !out->num_blocks) {
     const uint64_t want = 4ull * (uint64_t)(ctx->compute_units > 0 ? ctx->compute_units : 256);
     const uint64_t tpb = out->threads_per_block ? out->threads_per_block : 256;
@@ -165,6 +186,21 @@ static void load_tuning_file(q1_ctx* ctx) {
       if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL && shape_ok) {
         ctx->tuning[l][p].threads_per_block = tpb;
         ctx->tuning[l][p].tuples_per_thread = tpt;
+      }
+    } else if (kind == "launch_at") {
+      // launch_at <rows> <layout> <placement> <threads_per_block> <tuples_per_thread>
+      uint64_t rows = 0;
+      uint32_t tpb = 0, tpt = 0;
+      ls >> rows >> l >> p >> tpb >> tpt;
+      const bool shape_ok = (tpb == 64 || tpb == 128 || tpb == 256) && tpt && tpt % 16 == 0 && tpt <= 16384;
+      if (rows && valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL && shape_ok) {
+        std::vector<SizedLaunch>& at = ctx->tuning_at[l][p];
+        auto it = std::lower_bound(at.begin(), at.end(), rows,
+                                   [](const SizedLaunch& a, uint64_t r) { return a.rows < r; });
+        if (it != at.end() && it->rows == rows)
+          *it = SizedLaunch{rows, tpb, tpt};
+        else
+          at.insert(it, SizedLaunch{rows, tpb, tpt});
       }
     } else if (kind == "best") {
       ls >> l >> p;
@@ -761,12 +797,20 @@ q1_status q1_tuning_set(q1_ctx* ctx, q1_layout layout, q1_placement placement, c
   if (s != Q1_OK) return fail(ctx, s, why);
   ctx->tuning[layout][placement] = *best;
   ctx->tuning[layout][placement].num_blocks = 0;
+  ctx->tuning_at[layout][placement].clear();
   return Q1_OK;
 }
 
 q1_status q1_tuning_get(q1_ctx* ctx, q1_layout layout, q1_placement placement, q1_launch* out) {
   if (!ctx || !out || !valid_layout(layout) || !valid_placement(placement)) return Q1_EINVAL;
   *out = ctx->tuning[layout][resolve_placement(ctx, layout, placement)];
+  return Q1_OK;
+}
+
+q1_status q1_tuning_resolve(q1_ctx* ctx, q1_layout layout, q1_placement placement, uint64_t rows,
+                            q1_launch* out) {
+  if (!ctx || !out || !valid_layout(layout) || !valid_placement(placement)) return Q1_EINVAL;
+  resolve_launch(ctx, layout, resolve_placement(ctx, layout, placement), rows, nullptr, out);
   return Q1_OK;
 }
 

@@ -6,6 +6,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_set>
+#include <vector>
 
 #include "../../include/q1_hip.h"
 #include "q1_kernels.h"
@@ -25,6 +26,11 @@ struct q1_table {
 struct q1_batch_state;  // q1_batch.cpp
 namespace q1 {
 struct StreamState;     // q1_stream.cpp: the host-resident streaming pipeline
+// A launch shape tuned on ranges of `rows` rows ("launch_at" lines of the tuning file).
+struct SizedLaunch {
+  uint64_t rows;
+  uint32_t threads_per_block, tuples_per_thread;
+};
 }
 
 struct q1_ctx {
@@ -40,6 +46,9 @@ struct q1_ctx {
   std::string error;
   int32_t compute_units = 0;
   q1_launch tuning[4][5]{};                // [layout][placement]
+  // Shapes tuned at several row counts, ascending; a launch takes the one tuned nearest its
+  // own row count (by ratio) and falls back to tuning[][] when there is none.
+  std::vector<q1::SizedLaunch> tuning_at[4][5];
   // The batched kernel's tuned shape per [layout][placement]: num_blocks holds blocks per
   // compute unit, tuples_per_thread rows per lane; zero = not tuned.
   q1_launch batch_tuning[4][5]{};

@@ -185,6 +185,7 @@ def _load():
                                  C.POINTER(Result)]),
         "q1_tuning_set": (I, [P, I, I, C.POINTER(Launch)]),
         "q1_tuning_get": (I, [P, I, I, C.POINTER(Launch)]),
+        "q1_tuning_resolve": (I, [P, I, I, U64, C.POINTER(Launch)]),
         "q1_tuning_set_best_placement": (I, [P, I, I]),
     }
     optional = {
@@ -468,6 +469,12 @@ class Context:
     def tuning_get(self, layout: int, placement: int) -> Launch:
         l = Launch()
         self.check(lib.q1_tuning_get(self.handle, layout, placement, C.byref(l)), "q1_tuning_get")
+        return l
+
+    def tuning_resolve(self, layout: int, placement: int, rows: int) -> Launch:
+        """The shape a q1_run of `rows` rows with no caller shape takes."""
+        l = Launch()
+        self.check(lib.q1_tuning_resolve(self.handle, layout, placement, rows, C.byref(l)), "q1_tuning_resolve")
         return l
 
     def set_best_placement(self, layout: int, placement: int):
