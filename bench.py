@@ -358,9 +358,9 @@ def uncompressed_leg(q1, ctx, rows: int, seed: int, reps: int = 20):
 
 def host_leg(q1, ctx, table, rows: int, seed: int, want, cpu_threads: int, reps: int = 3):
     """Host-resident input, the reference's own mode: the same compressed rows generated
-    into pinned host memory.  (1) q1_run_streaming: chunks of 16M rows copied over 4
-    streams with the copies overlapping the kernels (src/main.cu:567-724), end-to-end time
-    per query.  (2) q1_run_split, the CPU/GPU coprocessing split (src/main.cu:513-550):
+    into pinned host memory.  (1) q1_run_streaming: chunks of the autotuning table's rows
+    per chunk copied over its number of streams, with the copies overlapping the kernels
+    (src/main.cu:567-724), end-to-end time per query.  (2) q1_run_split, the CPU/GPU coprocessing split (src/main.cu:513-550):
     the device scans [0, split) of the resident table while `cpu_threads` host threads
     aggregate [split, rows) of the host copy, at the reference's default CPU fraction 0.5
     and at the fraction that balances the two sides' measured rates.  Every result is
@@ -371,7 +371,7 @@ def host_leg(q1, ctx, table, rows: int, seed: int, want, cpu_threads: int, reps:
         q1.fill_host(q1.COMPRESSED, seed, cols, rows, 0, cpu_threads)
         gen_s = time.perf_counter() - t0
         nbytes = rows * q1.BITS_PER_ROW[q1.COMPRESSED] // 8
-        chunk, streams = 1 << 24, 4
+        chunk, streams = ctx.tuning_get_streaming(q1.COMPRESSED)
         dom = q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN)   # generated rows lie in the TPC-H domain
         exact = ctx.run_streaming(q1.COMPRESSED, cols, rows, 729999, q1.AUTO, dom, chunk, streams) == want
         walls = []

@@ -292,7 +292,9 @@ q1_status q1_run_split(q1_ctx* ctx, const q1_table* device_table, const q1_colum
                        int cpu_threads, q1_result* out, double* cpu_ms, double* gpu_ms);
 
 /* Host-resident columns streamed to the device in chunks over `num_streams` streams
- * with copies overlapping kernels (src/main.cu:567-724).  `host` should be pinned. */
+ * with copies overlapping kernels (src/main.cu:567-724).  `host` should be pinned.
+ * chunk_rows (a multiple of 32) / num_streams (1..16) of 0 take the autotuning table's
+ * values for the layout (q1_tuning_get_streaming). */
 q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host, uint64_t rows,
                            int32_t cutoff_day, q1_placement placement, const q1_launch* launch,
                            uint64_t chunk_rows, int num_streams, q1_result* out);
@@ -315,6 +317,9 @@ q1_status q1_tuning_get(q1_ctx* ctx, q1_layout layout, q1_placement placement, q
 q1_status q1_tuning_resolve(q1_ctx* ctx, q1_layout layout, q1_placement placement, uint64_t rows,
                             q1_launch* out);
 q1_status q1_tuning_set_best_placement(q1_ctx* ctx, q1_layout layout, q1_placement placement);
+/* The chunk rows and stream count q1_run_streaming takes for 0 ("streaming" lines of the
+ * table; else 16M rows and 4 streams). */
+q1_status q1_tuning_get_streaming(q1_ctx* ctx, q1_layout layout, uint64_t* chunk_rows, int* num_streams);
 
 /* --- serving (new): continuous batching over HBM-resident tables ------------------ */
 typedef struct q1_server q1_server;

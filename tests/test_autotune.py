@@ -16,7 +16,7 @@ def test_committed_tables_are_write_tables_output(tmp_path):
     with open(os.path.join(TUNING, "gfx950.json")) as f:
         table = json.load(f)
     autotune.write_tables(table["measurements"], str(tmp_path), table["device"], table["rows"],
-                          table.get("batch_measurements", []))
+                          table.get("batch_measurements", []), stream_results=table.get("streaming_measurements", []))
     for name in ("gfx950.txt", "gfx950.json"):
         with open(os.path.join(TUNING, name)) as a, open(tmp_path / name) as b:
             assert a.read() == b.read(), name
@@ -159,3 +159,29 @@ def test_resolve_takes_the_nearest_row_count(tmp_path, monkeypatch):
         tm = ctx.last_timing()
         assert (tm.threads_per_block, tm.tuples_per_thread) == shape(300_000)
         t.free()
+
+
+@pytest.mark.gpu
+def test_streaming_sweep_is_read_back(tmp_path, monkeypatch, oracle):
+    """A small sweep of the host-resident pipeline's rows per chunk x streams: its best pair
+    becomes the table's streaming line, q1_run_streaming with chunk_rows = streams = 0 runs
+    it (and matches the oracle), and a layout without a line keeps 16M rows x 4 streams."""
+    rows, seed = 3_000_017, 0x51C00044
+    with open(os.path.join(TUNING, "gfx950.json")) as f:
+        table = json.load(f)
+    with q1.Context(0) as ctx:
+        sres = autotune.sweep_streaming(ctx, q1.COMPRESSED, rows, 1, seed=seed, log=lambda s: None,
+                                        chunks=[1 << 19, 1 << 20], counts=[1, 3], host_threads=4)
+    assert len(sres) == 4 and all(r["wall_ms"] > 0 for r in sres)
+    autotune.write_tables(table["measurements"], str(tmp_path), table["device"], table["rows"],
+                          table.get("batch_measurements", []), stream_results=sres)
+    best = autotune.best_streaming(sres)[q1.COMPRESSED]
+    monkeypatch.setenv("Q1_TUNING_FILE", str(tmp_path / "gfx950.txt"))
+    with q1.Context(0) as ctx:
+        assert ctx.tuning_get_streaming(q1.COMPRESSED) == (best["chunk_rows"], best["streams"])
+        assert ctx.tuning_get_streaming(q1.UNCOMPRESSED) == (1 << 24, 4)
+        want, _ = oracle.naive(q1.generate_host(q1.REFERENCE, seed, rows), 0, rows, 729906)
+        got = ctx.run_streaming(q1.COMPRESSED, q1.generate_host(q1.COMPRESSED, seed, rows), rows, 729906,
+                                q1.AUTO, q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN), 0, 0)
+        assert [dict(sum_qty=g.sum_qty, sum_base_price=g.sum_base_price, sum_disc_price=g.sum_disc_price,
+                     sum_charge=g.sum_charge, sum_disc=g.sum_disc, count=g.count) for g in got] == want

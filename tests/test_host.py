@@ -151,7 +151,8 @@ def test_result_merge_is_exact():
 def test_shipped_tuning_table_is_well_formed():
     """The autotuning table that q1_init loads next to the library: every launch shape is
     one the kernels accept, every layout names a best placement, and the batched kernel's
-    shapes (compressed layout, both batch placements) are ones q1_init accepts."""
+    shapes (compressed layout, both batch placements) and the streaming pipeline's chunk
+    rows and stream counts are ones q1_init accepts."""
     import os
 
     from tpchq01_gpu_amd import PACKAGE_DIR
@@ -174,6 +175,9 @@ def test_shipped_tuning_table_is_well_formed():
             most = 256 if p == q1.PER_THREAD_SHARED_MEM else 16384
             assert 1 <= per_cu <= 64 and lane_rows % 16 == 0 and 16 <= lane_rows <= most, line
             batch[(l, p)] = (per_cu, lane_rows)
+        elif f[0] == "streaming":
+            l, chunk, streams = map(int, f[1:4])
+            assert 0 <= l <= 3 and chunk > 0 and chunk % 32 == 0 and 1 <= streams <= 16, line
         else:
             assert f[0] == "best", line
             best[int(f[1])] = int(f[2])

@@ -45,6 +45,8 @@ THREADS = [64, 128, 256]
 TUPLES = [16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
 ROWS = [600_000_000, 60_000_000, 6_000_000]
 BATCH_BLOCKS_PER_CU = [1, 2, 3, 4, 6, 8]
+STREAM_CHUNKS = [1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24, 1 << 25, 1 << 26]
+STREAM_COUNTS = [1, 2, 3, 4, 6, 8]
 BATCH_LANE_ROWS = {q1.IN_REGISTERS: [256, 512, 1024, 2048], q1.PER_THREAD_SHARED_MEM: [64, 128, 192, 256]}
 MALL_BYTES = 256 << 20
 HBM_PEAK_GBS = 8000.0
@@ -127,6 +129,39 @@ def sizes_of(results):
     return sorted({r["rows"] for r in results}, reverse=True)
 
 
+def sweep_streaming(ctx: q1.Context, layout: int, rows: int, reps: int, seed: int = 0x51C00011, log=print,
+                    chunks=STREAM_CHUNKS, counts=STREAM_COUNTS, host_threads: int = 16):
+    """The host-resident pipeline (q1_run_streaming): rows per chunk x streams, on `rows`
+    rows generated into pinned host memory, end-to-end wall time per query (the minimum
+    over repetitions after one untimed call that sets up the staging buffers); the
+    reference sweeps the same two knobs (--tuples-per-kernel-launch, --streams,
+    scripts/try_grid_params.py:37-38, scripts/plots.R's streams plot)."""
+    algo = rows * q1.BITS_PER_ROW[layout] // 8
+    cols = ctx.pinned_columns(layout, rows)
+    results = []
+    try:
+        q1.fill_host(layout, seed, cols, rows, 0, host_threads)
+        dom = q1.make_launch(flags=q1.LAUNCH_ASSUME_TPCH_DOMAIN)   # generated rows lie in the TPC-H domain
+        for chunk in chunks:
+            for streams in counts:
+                if chunk > rows:
+                    continue
+                best = float("inf")
+                for r in range(reps + 1):
+                    t0 = time.perf_counter()
+                    ctx.run_streaming(layout, cols, rows, 729999, q1.AUTO, dom, chunk, streams)
+                    if r:
+                        best = min(best, (time.perf_counter() - t0) * 1e3)
+                gbs = algo / (best * 1e-3) / 1e9
+                results.append(dict(layout=layout, placement=q1.AUTO, rows=rows, chunk_rows=chunk, streams=streams,
+                                    wall_ms=best, kernel_ms=best, gbps=gbs))
+                log(f"streaming {q1.LAYOUT_NAMES[layout]:>20} chunk={chunk:9d} streams={streams} {best:8.2f} ms "
+                    f"{gbs:6.1f} GB/s")
+    finally:
+        ctx.free_pinned_columns(cols)
+    return results
+
+
 def best_table(results, rows=None):
     """The fastest shape per (layout, placement) at `rows` rows (default: the largest row
     count measured), and the fastest placement per layout among those."""
@@ -153,13 +188,24 @@ def merge_results(old, new):
     return [r for r in old if (r["layout"], r["placement"], r["rows"]) not in swept] + list(new)
 
 
-def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arch: str = "gfx950"):
+def best_streaming(stream_results):
+    """The fastest (chunk rows, streams) per layout."""
+    best = {}
+    for r in stream_results:
+        if r["layout"] not in best or r["wall_ms"] < best[r["layout"]]["wall_ms"]:
+            best[r["layout"]] = r
+    return best
+
+
+def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arch: str = "gfx950",
+                 stream_results=()):
     """Writes tuning/<arch>.txt (read by q1_init) and tuning/<arch>.json.  `rows` (one row
     count or a list) is recorded as the sweep's; the lines come from the measurements."""
     os.makedirs(out_dir, exist_ok=True)
     sizes = sizes_of(results)
     best, per_layout = best_table(results)
     bbest, _ = best_table(batch_results)
+    sbest = best_streaming(stream_results)
     with open(os.path.join(out_dir, f"{arch}.txt"), "w") as f:
         swept = ", ".join(str(n) for n in sizes)
         f.write(f"# written by tpchq01_gpu_amd.autotune on {device}, every layout swept at {swept} "
@@ -169,6 +215,8 @@ def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arc
             f.write("#   launch_at <rows> <layout> <placement> <threads_per_block> <tuples_per_thread>"
                     "   (a launch takes the row count nearest its own)\n")
         f.write("#   batch <layout> <placement> <blocks_per_cu> <tuples_per_thread>   (the batched kernel, 256 threads)\n")
+        if sbest:
+            f.write("#   streaming <layout> <chunk_rows> <streams>   (host-resident columns, q1_run_streaming)\n")
         f.write("# (layouts 0 reference, 1 uncompressed, 2 compressed, 3 compressed + pushdown bitmap)\n")
         for (l, p), r in sorted(best.items()):
             f.write(f"launch {l} {p} {r['tpb']} {r['tpt']}\n")
@@ -180,9 +228,12 @@ def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arc
             f.write(f"best {l} {r['placement']}\n")
         for (l, p), r in sorted(bbest.items()):
             f.write(f"batch {l} {p} {r['blocks_per_cu']} {r['tpt']}\n")
+        for l, r in sorted(sbest.items()):
+            f.write(f"streaming {l} {r['chunk_rows']} {r['streams']}\n")
     with open(os.path.join(out_dir, f"{arch}.json"), "w") as f:
+        extra = {"streaming_measurements": list(stream_results)} if stream_results else {}
         json.dump({"device": device, "rows": rows, "measurements": list(results),
-                   "batch_measurements": list(batch_results),
+                   "batch_measurements": list(batch_results), **extra,
                    "best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
                             for n in sizes for (l, p), r in sorted(best_table(results, n)[0].items())],
                    "batch_best": [{**r, "layout": q1.LAYOUT_NAMES[l], "placement": q1.PLACEMENT_NAMES[p]}
@@ -215,6 +266,21 @@ def report(table: dict) -> str:
             lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]} | {r['blocks_per_cu']} | {r['tpt']} | "
                          f"{r['requests']} | {r['rows']} | {r['kernel_ms']:.3f} | {r['gbps']:.0f} | "
                          f"{100 * r['gbps'] / HBM_PEAK_GBS:.1f} |")
+    sres = table.get("streaming_measurements", [])
+    if sres:
+        lines += ["", "Host-resident columns streamed to the device (`q1_run_streaming`, end-to-end ms per query):"]
+        for l, b in sorted(best_streaming(sres).items()):
+            mine = [r for r in sres if r["layout"] == l]
+            chunks = sorted({r["chunk_rows"] for r in mine})
+            counts = sorted({r["streams"] for r in mine})
+            ms = {(r["chunk_rows"], r["streams"]): r["wall_ms"] for r in mine}
+            lines += ["", f"{q1.LAYOUT_NAMES[l]}, {b['rows']:,} rows; best {b['chunk_rows']:,} rows per chunk x "
+                          f"{b['streams']} streams, {b['wall_ms']:.1f} ms ({b['gbps']:.1f} GB/s):", "",
+                      "| rows per chunk | " + " | ".join(f"{c} streams" for c in counts) + " |",
+                      "|---|" + "---|" * len(counts)]
+            for ch in chunks:
+                cells = [f"{ms[(ch, c)]:.1f}" if (ch, c) in ms else "" for c in counts]
+                lines.append(f"| {ch:,} | " + " | ".join(cells) + " |")
     return "\n".join(lines) + "\n"
 
 
@@ -227,6 +293,9 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--batch", action="store_true", help="also sweep the batched kernel's shapes (compressed)")
     ap.add_argument("--batch-requests", type=int, default=256)
+    ap.add_argument("--streaming", action="store_true",
+                    help="also sweep the host-resident pipeline's rows per chunk x streams (compressed)")
+    ap.add_argument("--streaming-rows", type=int, default=600_000_000)
     ap.add_argument("--merge", action="store_true",
                     help="keep the existing table's measurements of what this run does not sweep")
     ap.add_argument("--report", action="store_true", help="print (and write) the report of the existing table")
@@ -242,7 +311,7 @@ def main(argv=None):
         return 0
     layouts = [LAYOUT_KEYS[x] for x in args.layouts.split(",") if x]
     placements = [q1.PLACEMENTS[x] for x in args.placements.split(",")]
-    old = {"measurements": [], "batch_measurements": []}
+    old = {"measurements": [], "batch_measurements": [], "streaming_measurements": []}
     if args.merge and os.path.exists(json_path):
         with open(json_path) as f:
             old = json.load(f)
@@ -261,9 +330,14 @@ def main(argv=None):
             from . import workloads
             batch_results = sweep_batch(ctx, q1.COMPRESSED, workloads.batch_spec(args.batch_requests),
                                         [q1.IN_REGISTERS, q1.PER_THREAD_SHARED_MEM], args.reps, log=out)
+        stream_results = []
+        if args.streaming:
+            stream_results = sweep_streaming(ctx, q1.COMPRESSED, args.streaming_rows, args.reps, log=out)
     results = merge_results(old.get("measurements", []), results)
     batch_results = merge_results(old.get("batch_measurements", []), batch_results)
-    best, per_layout = write_tables(results, args.out, device, rows if len(rows) > 1 else rows[0], batch_results)
+    stream_results = merge_results(old.get("streaming_measurements", []), stream_results)
+    best, per_layout = write_tables(results, args.out, device, rows if len(rows) > 1 else rows[0], batch_results,
+                                    stream_results=stream_results)
     for l, r in per_layout.items():
         print(f"best {q1.LAYOUT_NAMES[l]}: {q1.PLACEMENT_NAMES[r['placement']]} tpb={r['tpb']} tpt={r['tpt']} "
               f"{r['kernel_ms']:.3f} ms {r['gbps']:.1f} GB/s", flush=True)

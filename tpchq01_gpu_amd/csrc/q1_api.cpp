@@ -202,6 +202,15 @@ static void load_tuning_file(q1_ctx* ctx) {
         else
           at.insert(it, SizedLaunch{rows, tpb, tpt});
       }
+    } else if (kind == "streaming") {
+      // streaming <layout> <chunk rows> <streams>
+      uint64_t chunk = 0;
+      int streams = 0;
+      ls >> l >> chunk >> streams;
+      if (valid_layout(l) && chunk && chunk % 32 == 0 && streams >= 1 && streams <= 16) {
+        ctx->stream_chunk_rows[l] = chunk;
+        ctx->stream_count[l] = streams;
+      }
     } else if (kind == "best") {
       ls >> l >> p;
       if (valid_layout(l) && p >= 0 && p <= Q1_PLACE_GLOBAL) ctx->best[l] = (q1_placement)p;
@@ -811,6 +820,13 @@ q1_status q1_tuning_resolve(q1_ctx* ctx, q1_layout layout, q1_placement placemen
                             q1_launch* out) {
   if (!ctx || !out || !valid_layout(layout) || !valid_placement(placement)) return Q1_EINVAL;
   resolve_launch(ctx, layout, resolve_placement(ctx, layout, placement), rows, nullptr, out);
+  return Q1_OK;
+}
+
+q1_status q1_tuning_get_streaming(q1_ctx* ctx, q1_layout layout, uint64_t* chunk_rows, int* num_streams) {
+  if (!ctx || !chunk_rows || !num_streams || !valid_layout(layout)) return Q1_EINVAL;
+  *chunk_rows = ctx->stream_chunk_rows[layout] ? ctx->stream_chunk_rows[layout] : kStreamChunkRows;
+  *num_streams = ctx->stream_count[layout] ? ctx->stream_count[layout] : kStreamCount;
   return Q1_OK;
 }
 

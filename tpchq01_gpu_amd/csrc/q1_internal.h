@@ -23,6 +23,11 @@ struct q1_table {
   int32_t filter_cutoff = 0;     // pushdown tables: the cutoff their bitmap encodes
 };
 
+// q1_run_streaming's chunk rows and stream count when neither the caller nor the tuning
+// table gives them.
+constexpr uint64_t kStreamChunkRows = 1ull << 24;
+constexpr int kStreamCount = 4;
+
 struct q1_batch_state;  // q1_batch.cpp
 namespace q1 {
 struct StreamState;     // q1_stream.cpp: the host-resident streaming pipeline
@@ -52,6 +57,10 @@ struct q1_ctx {
   // The batched kernel's tuned shape per [layout][placement]: num_blocks holds blocks per
   // compute unit, tuples_per_thread rows per lane; zero = not tuned.
   q1_launch batch_tuning[4][5]{};
+  // The host-resident streaming pipeline's tuned chunk rows and stream count per layout
+  // (0 = not tuned: kStreamChunkRows, kStreamCount).
+  uint64_t stream_chunk_rows[4] = {};
+  int stream_count[4] = {};
   q1_placement best[4] = {Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS, Q1_PLACE_IN_REGISTERS,
                           Q1_PLACE_IN_REGISTERS};
   q1_batch_state* batch = nullptr;

@@ -101,7 +101,10 @@ q1_status q1_run_streaming(q1_ctx* ctx, q1_layout layout, const q1_columns* host
                            q1_result* out) {
   if (!ctx || !host || !out || layout < Q1_LAYOUT_REFERENCE || layout > Q1_LAYOUT_COMPRESSED_PUSHDOWN)
     return fail(ctx, Q1_EINVAL, "bad argument");
-  if (chunk_rows == 0 || chunk_rows % 32 || num_streams < 1 || num_streams > 16)
+  // 0 = the autotuning table's value for the layout ("streaming" lines), else 16M rows / 4 streams
+  if (chunk_rows == 0) chunk_rows = ctx->stream_chunk_rows[layout] ? ctx->stream_chunk_rows[layout] : kStreamChunkRows;
+  if (num_streams == 0) num_streams = ctx->stream_count[layout] ? ctx->stream_count[layout] : kStreamCount;
+  if (chunk_rows % 32 || num_streams < 1 || num_streams > 16)
     return fail(ctx, Q1_EINVAL, "chunk_rows must be a positive multiple of 32 and streams in [1, 16]");
   try {
     std::lock_guard<std::mutex> g(ctx->lock);

@@ -187,6 +187,7 @@ def _load():
         "q1_tuning_get": (I, [P, I, I, C.POINTER(Launch)]),
         "q1_tuning_resolve": (I, [P, I, I, U64, C.POINTER(Launch)]),
         "q1_tuning_set_best_placement": (I, [P, I, I]),
+        "q1_tuning_get_streaming": (I, [P, I, C.POINTER(U64), C.POINTER(I)]),
     }
     optional = {
         "q1_run_batch": (I, [P, C.POINTER(Request), C.c_uint32, C.POINTER(Launch), C.POINTER(Result)]),
@@ -454,7 +455,7 @@ class Context:
         return result_to_list(r), cms.value, gms.value
 
     def run_streaming(self, layout: int, host_cols: dict, rows: int, cutoff_day: int = DEFAULT_CUTOFF,
-                      placement=IN_REGISTERS, launch=None, chunk_rows: int = 1 << 21, streams: int = 4):
+                      placement=IN_REGISTERS, launch=None, chunk_rows: int = 0, streams: int = 0):
         st = columns_struct(host_cols)
         r = Result()
         self.check(lib.q1_run_streaming(self.handle, layout, C.byref(st), rows, cutoff_day, placement,
@@ -476,6 +477,13 @@ class Context:
         l = Launch()
         self.check(lib.q1_tuning_resolve(self.handle, layout, placement, rows, C.byref(l)), "q1_tuning_resolve")
         return l
+
+    def tuning_get_streaming(self, layout: int):
+        """(chunk_rows, streams) that q1_run_streaming takes when given 0."""
+        chunk, streams = C.c_uint64(), C.c_int()
+        self.check(lib.q1_tuning_get_streaming(self.handle, layout, C.byref(chunk), C.byref(streams)),
+                   "q1_tuning_get_streaming")
+        return chunk.value, streams.value
 
     def set_best_placement(self, layout: int, placement: int):
         self.check(lib.q1_tuning_set_best_placement(self.handle, layout, placement),
