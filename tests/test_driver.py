@@ -30,11 +30,92 @@ def run(args, cwd=None, timeout=300):
     (["--streams=0"], "--streams"),
     (["--tuples-per-kernel-launch=100"], "multiple of 32"),
     (["--cutoff=1998-13-01"], "Cannot parse the date"),
+    (["--cutoff=1997-02-29"], "Cannot parse the date"),
 ])
 def test_bad_options_exit_with_failure(args, msg):
     r = run(args)
     assert r.returncode == 1
     assert msg in r.stderr
+
+
+# lineitem.tbl lines in dbgen's text form (16 '|'-terminated fields; quantity printed without
+# decimals, as dbgen does, or with them), including a leap day, the first and last ship
+# dates of the generator's range and the value extremes.
+DBGEN_LINES = [
+    "1|155190|7706|1|17|21168.23|0.04|0.02|N|O|1996-03-13|1996-02-12|1996-03-22|DELIVER IN PERSON|TRUCK|egular courts above the|",
+    "1|67310|7311|2|36|45983.16|0.09|0.06|N|O|1996-04-12|1996-02-28|1996-04-20|TAKE BACK RETURN|MAIL|ly final dependencies: slyly bold |",
+    "3|4297|1798|1|45|54058.05|0.06|0.00|R|F|1994-02-02|1994-01-04|1994-02-23|NONE|AIR|ongside of the furiously brave acco|",
+    "32|82704|7721|1|28|47227.60|0.05|0.08|N|O|1995-10-23|1995-08-27|1995-10-26|TAKE BACK RETURN|TRUCK|sleep quickly. req|",
+    "35|1|1|1|24|22824.48|0.02|0.00|A|F|1992-02-29|1992-03-01|1992-03-02|NONE|AIR|a leap day|",
+    "36|1|1|1|50.00|104999.50|0.10|0.08|R|F|1998-12-01|1998-12-01|1998-12-31|NONE|SHIP|the last ship date|",
+    "37|1|1|1|1|900.00|0.00|0.00|A|F|1992-01-02|1992-01-02|1992-01-03|NONE|RAIL|the first ship date|",
+    "38|1|1|1|2.5|1800.9|0.1|0.01|N|F|1995-06-17|1995-06-17|1995-06-18|NONE|FOB|one fraction digit|",
+]
+
+
+def expected_columns(lines):
+    """The reference-layout columns (src/data_types.h:19-25) that the text fixes, computed
+    from the characters of each line: decimals x100, MonetDB day numbers (1992-01-01 is
+    727563, src/constants.hpp:166-186)."""
+    import datetime
+    from decimal import Decimal
+    base = datetime.date(1992, 1, 1)
+    cols = {k: [] for k in ("ship_date", "quantity", "extended_price", "discount", "tax", "return_flag",
+                            "line_status")}
+    for line in lines:
+        f = line.split("|")
+        for k, i in (("quantity", 4), ("extended_price", 5), ("discount", 6), ("tax", 7)):
+            cols[k].append(int(Decimal(f[i]) * 100))
+        cols["return_flag"].append(ord(f[8]))
+        cols["line_status"].append(ord(f[9]))
+        y, m, d = map(int, f[10].split("-"))
+        cols["ship_date"].append(727563 + (datetime.date(y, m, d) - base).days)
+    return cols
+
+
+@pytest.mark.parametrize("eol", ["\n", "\r\n"])
+def test_ingest_only_parses_dbgen_text(tmp_path, eol):
+    """--ingest-only parses lineitem.tbl into the reference's column cache without a GPU;
+    every column equals what the text of the lines fixes."""
+    d = tmp_path / "data" / "1.000000"
+    d.mkdir(parents=True)
+    with open(d / "lineitem.tbl", "w", newline="") as f:
+        f.write(eol.join(DBGEN_LINES) + eol)
+    r = run(["--data-dir=%s" % (tmp_path / "data"), "--ingest-only"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "table length: %d records" % len(DBGEN_LINES) in r.stdout
+    want = expected_columns(DBGEN_LINES)
+    assert want["ship_date"][4] == 727563 + 59 and want["quantity"][7] == 250 and want["extended_price"][7] == 180090
+    for name, col, dt in (("shipdate", "ship_date", np.int32), ("quantity", "quantity", np.int64),
+                          ("extendedprice", "extended_price", np.int64), ("discount", "discount", np.int64),
+                          ("tax", "tax", np.int64), ("returnflag", "return_flag", np.int8),
+                          ("linestatus", "line_status", np.int8)):
+        got = np.fromfile(d / f"{name}.bin", dt)
+        assert got.tolist() == want[col], name
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ("1|1|1|1|1x|900.00|0.00|0.00|A|F|1992-01-02|x|x|x|x|x|", "bad decimal"),
+    ("1|1|1|1|1|900.001|0.00|0.00|A|F|1992-01-02|x|x|x|x|x|", "bad decimal"),
+    ("1|1|1|1|1|900.00|.5|0.00|A|F|1992-01-02|x|x|x|x|x|", "bad decimal"),
+    ("1|1|1|1|1|900.00|0.00|0.00|A|F|1995-02-29|x|x|x|x|x|", "bad ship date"),
+    ("1|1|1|1|1|900.00|0.00|0.00|A|F|1992-1-02|x|x|x|x|x|", "bad ship date"),
+    ("1|1|1|1|1|900.00|0.00|0.00|AF|F|1992-01-02|x|x|x|x|x|", "bad return flag"),
+    ("1|1|1|1|1|900.00|0.00|0.00|A|F", "fewer than 11 fields"),
+])
+def test_ingest_only_rejects_malformed_lines(tmp_path, bad, msg):
+    d = tmp_path / "data" / "1.000000"
+    d.mkdir(parents=True)
+    (d / "lineitem.tbl").write_text(DBGEN_LINES[0] + "\n" + bad + "\n")
+    r = run(["--data-dir=%s" % (tmp_path / "data"), "--ingest-only"], cwd=tmp_path)
+    assert r.returncode == 1
+    assert msg in r.stderr and "line 2" in r.stderr
+    assert not (d / "shipdate.bin").exists()
+
+
+def test_ingest_only_needs_a_data_dir():
+    r = run(["--ingest-only"])
+    assert r.returncode == 1 and "--data-dir" in r.stderr
 
 
 def parse_printed(out: str):

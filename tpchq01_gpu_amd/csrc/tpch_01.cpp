@@ -18,6 +18,8 @@
 //   --seed=N --cutoff=YYYY-MM-DD --cpu-threads=N
 //   --filter-once        with --use-filter-pushdown: build the filter bitmap once, before
 //                        the timed runs (off the critical path), instead of in every run
+//   --ingest-only        with --data-dir: parse DIR/<sf>/lineitem.tbl, (re)write the
+//                        column cache and exit, without touching the GPU
 // Each run's wall time goes to results.csv (src/main.cu:881-905).
 #include <sys/stat.h>
 
@@ -58,6 +60,7 @@ struct Params {
   int cpu_threads = 0;
   bool user_threads = false;
   bool filter_once = false;
+  bool ingest_only = false;
 };
 
 [[noreturn]] void usage(const char* argv0) {
@@ -71,7 +74,8 @@ struct Params {
                "   --runs=[default:5]\n   --sf=|--scale-factor=[default:1]\n   --rows=N\n"
                "   --cpu-fraction=[default:0.5]\n   --streams=[default:4]\n"
                "   --threads-per-block=N\n   --tuples-per-thread=N\n   --tuples-per-kernel-launch=[default:2097152]\n"
-               "   --data-dir=DIR\n   --seed=N\n   --cutoff=YYYY-MM-DD\n   --cpu-threads=N\n   --filter-once\n",
+               "   --data-dir=DIR\n   --seed=N\n   --cutoff=YYYY-MM-DD\n   --cpu-threads=N\n   --filter-once\n"
+               "   --ingest-only\n",
                argv0);
   std::exit(EXIT_FAILURE);
 }
@@ -86,9 +90,15 @@ int placement_code(const std::string& s) {
   return -1;
 }
 
+// "YYYY-MM-DD" exactly (the .tbl date form), a real calendar day -> MonetDB day number.
 bool parse_date(const std::string& s, int32_t* out) {
-  int y, m, d;
-  if (std::sscanf(s.c_str(), "%d-%d-%d", &y, &m, &d) != 3 || m < 1 || m > 12 || d < 1 || d > 31) return false;
+  if (s.size() != 10 || s[4] != '-' || s[7] != '-') return false;
+  for (int i : {0, 1, 2, 3, 5, 6, 8, 9})
+    if (s[i] < '0' || s[i] > '9') return false;
+  const int y = std::stoi(s.substr(0, 4)), m = std::stoi(s.substr(5, 2)), d = std::stoi(s.substr(8, 2));
+  static const int kDays[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+  const bool leap = (y % 4 == 0 && y % 100 != 0) || y % 400 == 0;
+  if (m < 1 || m > 12 || d < 1 || d > kDays[m - 1] + (m == 2 && leap)) return false;
   *out = q1_day_number(y, m, d);
   return true;
 }
@@ -120,6 +130,7 @@ Params parse(int argc, char** argv, bool* want_device) {
       else if (name == "print-results") p.print_results = true;
       else if (name == "device-resident") p.device_resident = true;
       else if (name == "filter-once") p.filter_once = true;
+      else if (name == "ingest-only") p.ingest_only = true;
       else if (name == "scale-factor" || name == "sf") {
         p.scale_factor = std::stod(value);
         if (p.scale_factor < 0.001) throw std::invalid_argument("Invalid scale factor " + value);
@@ -179,15 +190,21 @@ Params parse(int argc, char** argv, bool* want_device) {
     std::fprintf(stderr, "--tuples-per-kernel-launch must be a positive multiple of 32\n");
     std::exit(EXIT_FAILURE);
   }
+  if (p.ingest_only && p.data_dir.empty()) {
+    std::fprintf(stderr, "--ingest-only needs --data-dir\n");
+    std::exit(EXIT_FAILURE);
+  }
   if (p.rows == 0) p.rows = (uint64_t)(6001215.0 * p.scale_factor);
   return p;
 }
 
 // ---- host columns --------------------------------------------------------------------
+// Host columns of one layout: pinned through the context, or plain pageable memory when
+// there is no context (--ingest-only, which never touches the GPU).
 struct HostColumns {
   q1_layout layout;
   uint64_t rows = 0;
-  std::vector<void*> bufs;  // pinned
+  std::vector<void*> bufs;
   q1_columns cols{};
   q1_ctx* ctx = nullptr;
   HostColumns(q1_ctx* c, q1_layout l, uint64_t n) : layout(l), rows(n), ctx(c) {
@@ -197,14 +214,19 @@ struct HostColumns {
                              &cols.tax, &cols.return_flag, &cols.line_status, &cols.filter};
     for (int i = 0; i < 8; i++) {
       if (!b[i]) continue;
-      void* p = q1_host_alloc(ctx, b[i]);
-      if (!p) throw std::runtime_error("pinned host allocation failed");
+      void* p = ctx ? q1_host_alloc(ctx, b[i]) : std::aligned_alloc(64, (b[i] + 63) / 64 * 64);
+      if (!p) throw std::runtime_error("host column allocation failed");
       bufs.push_back(p);
       *slots[i] = p;
     }
   }
   ~HostColumns() {
-    for (void* p : bufs) q1_host_free(ctx, p);
+    for (void* p : bufs) {
+      if (ctx)
+        q1_host_free(ctx, p);
+      else
+        std::free(p);
+    }
   }
 };
 
@@ -252,20 +274,32 @@ void write_bin_cache(const Params& p, const HostColumns& h) {
   }
 }
 
-// Decimal(15,2) text -> value x100 (cpu/tpch_kit.hpp:41-82 semantics for "int.frac").
-int64_t parse_decimal(const char* s, const char* e) {
+// Decimal(15,2) text -> value x100 (cpu/tpch_kit.hpp:41-82 semantics for "int.frac"):
+// an optional '-', at least one digit, then optionally '.' and one or two digits
+// ("17" -> 1700, "0.1" -> 10, "21168.23" -> 2116823).  False for anything else.
+bool parse_decimal(const char* s, const char* e, int64_t* out) {
   bool neg = false;
   if (s < e && *s == '-') { neg = true; s++; }
   int64_t ip = 0, fp = 0;
-  int fd = 0;
-  while (s < e && *s != '.') ip = ip * 10 + (*s++ - '0');
+  int id = 0, fd = 0;
+  for (; s < e && *s >= '0' && *s <= '9'; s++, id++) {
+    if (id >= 15) return false;
+    ip = ip * 10 + (*s - '0');
+  }
+  if (id == 0) return false;
   if (s < e && *s == '.') {
     s++;
-    while (s < e && fd < 2) { fp = fp * 10 + (*s++ - '0'); fd++; }
+    for (; s < e && *s >= '0' && *s <= '9'; s++, fd++) {
+      if (fd >= 2) return false;
+      fp = fp * 10 + (*s - '0');
+    }
+    if (fd == 0) return false;
   }
+  if (s != e) return false;
   if (fd == 1) fp *= 10;
   const int64_t v = ip * 100 + fp;
-  return neg ? -v : v;
+  *out = neg ? -v : v;
+  return true;
 }
 
 // lineitem.tbl (dbgen, '|'-separated): fields 5-11 are quantity, extendedprice, discount,
@@ -277,7 +311,10 @@ std::unique_ptr<HostColumns> parse_tbl(q1_ctx* ctx, const std::string& path) {
   std::vector<int64_t> qty, price, disc, tax;
   std::vector<char> rf, ls;
   std::string line;
+  uint64_t line_no = 0;
   while (std::getline(in, line)) {
+    line_no++;
+    if (!line.empty() && line.back() == '\r') line.pop_back();
     const char* f[16];
     const char* fe[16];
     int k = 0;
@@ -290,15 +327,21 @@ std::unique_ptr<HostColumns> parse_tbl(q1_ctx* ctx, const std::string& path) {
         if (*c == 0) break;
       }
     }
-    if (k < 11) continue;
-    qty.push_back(parse_decimal(f[4], fe[4]));
-    price.push_back(parse_decimal(f[5], fe[5]));
-    disc.push_back(parse_decimal(f[6], fe[6]));
-    tax.push_back(parse_decimal(f[7], fe[7]));
+    if (k == 1 && fe[0] == f[0]) continue;  // empty line
+    const std::string where = path + " line " + std::to_string(line_no);
+    if (k < 11) throw std::runtime_error("fewer than 11 fields in " + where);
+    int64_t v[4];
+    for (int c = 0; c < 4; c++)
+      if (!parse_decimal(f[4 + c], fe[4 + c], &v[c])) throw std::runtime_error("bad decimal in " + where);
+    if (fe[8] - f[8] != 1 || fe[9] - f[9] != 1) throw std::runtime_error("bad return flag or line status in " + where);
+    qty.push_back(v[0]);
+    price.push_back(v[1]);
+    disc.push_back(v[2]);
+    tax.push_back(v[3]);
     rf.push_back(*f[8]);
     ls.push_back(*f[9]);
     int32_t d = 0;
-    if (!parse_date(std::string(f[10], fe[10]), &d)) throw std::runtime_error("bad ship date in " + path);
+    if (!parse_date(std::string(f[10], fe[10]), &d)) throw std::runtime_error("bad ship date in " + where);
     ship.push_back(d);
   }
   const uint64_t n = ship.size();
@@ -357,6 +400,20 @@ void check(q1_ctx* ctx, q1_status s, const char* what) {
 int main(int argc, char** argv) {
   bool want_device = false;
   const Params p = parse(argc, argv, &want_device);
+  if (p.ingest_only) {
+    // .tbl -> column cache on the host only (q1_init is never called)
+    try {
+      auto ref = parse_tbl(nullptr, sf_dir(p) + "/lineitem.tbl");
+      if (!ref) throw std::runtime_error("Cannot locate table text file " + sf_dir(p) + "/lineitem.tbl");
+      std::printf("CSV read & parsed; table length: %llu records.\n", (unsigned long long)ref->rows);
+      write_bin_cache(p, *ref);
+      std::printf("Column cache written to %s\n", sf_dir(p).c_str());
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "%s\n", e.what());
+      return EXIT_FAILURE;
+    }
+    return 0;
+  }
   q1_ctx* ctx = nullptr;
   const q1_status st = q1_init(0, &ctx);
   if (st != Q1_OK) {
