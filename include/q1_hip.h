@@ -219,7 +219,7 @@ q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout,
 /* No counterpart in the reference: q1_enqueue that allocates nothing (no hipMalloc, hipFree
  * or allocating library call), for callers that cannot allocate device memory -- code
  * under HIP graph capture, serving loops that forbid allocation.  Same arguments, results
- * and errors as q1_enqueue.  Its scratch (the spread-merge tables of grids of 1024 blocks or
+ * and errors as q1_enqueue.  Its scratch (the spread-merge tables of grids of 176 blocks or
  * more, and the global placement's per-block tables) is a static array of 8 MiB + 24 KiB
  * per GPU, set up by q1_init and held until the process ends; a launch needing more (the
  * global placement with more than 34952 blocks) gets Q1_EINVAL.  Every call on one GPU

@@ -113,8 +113,8 @@ def test_launch_shapes(ctx, oracle, shape):
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_spread_merge_large_grids(ctx, oracle, layout):
-    """Grids of 1024 blocks or more merge through the context's spread tables and a fold
-    kernel; launches one after another must each see clean tables."""
+    """Grids of kSpreadMinBlocks (176) blocks or more merge through the context's spread
+    tables and a fold kernel; launches one after another must each see clean tables."""
     rows = 2_000_003
     ref = host_tables(rows)[q1.REFERENCE]
     cutoff = 729906
@@ -135,9 +135,14 @@ def test_spread_merge_large_grids(ctx, oracle, layout):
     t.free()
 
 
+SPREAD_MIN_BLOCKS = 176   # kSpreadMinBlocks of the library
+
+
 def test_spread_merge_threshold(ctx, oracle):
-    """1023 blocks merge directly, 1024 through the spread tables: same answers."""
-    for rows in (1023 * 1024, 1023 * 1024 + 1, 1024 * 1024):
+    """Grids of 175 blocks merge directly, 176 and more through the spread tables: same
+    answers (64 lanes x 16 rows = 1024 rows per block)."""
+    k = SPREAD_MIN_BLOCKS
+    for rows in (1, (k - 1) * 1024, (k - 1) * 1024 + 1, k * 1024, 1023 * 1024 + 1):
         ref = q1.generate_host(q1.REFERENCE, SEED + 3, rows)
         comp = q1.generate_host(q1.COMPRESSED, SEED + 3, rows)
         want, _ = oracle.naive(ref, 0, rows, 729999)
@@ -294,7 +299,7 @@ def test_int128_carry_fast_path(ctx):
     t = ctx.upload(q1.COMPRESSED, cols, n)
     charge = n * (100 - disc) * price * (100 + tax)
     assert charge > 2**64
-    # the tuned shape merges directly; 64 x 64 gives 24415 blocks and the spread merge
+    # the tuned shapes and 64 x 64 (24415 blocks) merge through the spread tables
     for p, launch in ((q1.IN_REGISTERS, None), (q1.PER_THREAD_SHARED_MEM, None), (q1.IN_REGISTERS_PER_THREAD, None),
                       (q1.IN_REGISTERS, q1.make_launch(64, 64)), (q1.PER_THREAD_SHARED_MEM, q1.make_launch(64, 64))):
         g = ctx.run(t, 0, n, 729999, p, launch)[1 << 1 | 1]

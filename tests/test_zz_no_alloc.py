@@ -1,7 +1,7 @@
 """q1_enqueue_noalloc, the drop-in launch that allocates nothing (its scratch is a static
 per-device array set up by q1_init), against q1_enqueue: the same device table after two
 calls in a row, the second on a smaller input, for every placement, with grids that merge
-through the spread tables (>= 1024 blocks) and the global placement's per-block tables;
+through the spread tables (>= 176 blocks) and the global placement's per-block tables;
 launches that need more scratch than the array holds are refused."""
 import ctypes as C
 import sys

@@ -45,7 +45,12 @@ struct ScanArgs {
 // leaves them zero) and must not be shared by launches that can overlap: one set per
 // stream that serialises its launches.
 constexpr uint32_t kSpreadTables = 64;      // power of two
-constexpr uint32_t kSpreadMinBlocks = 1024; // smaller grids merge directly
+// Smaller grids merge directly.  Device-scope atomics on one address from all compute
+// units serialise: a grid's direct merge costs about 0.03 us per block against the fold's
+// fixed 4.5 us, so direct merging wins below about 165-190 blocks (measured on 5K-6M-row
+// launches: 49 blocks 11 us direct / 15 us spread, 489 blocks 28 / 15 us, 977 blocks 49 /
+// 16 us).
+constexpr uint32_t kSpreadMinBlocks = 176;
 struct Spread {
   q1_device_aggregates* tables;  // kSpreadTables zeroed tables
 };
