@@ -211,7 +211,10 @@ q1_status q1_last_timing(q1_ctx* ctx, q1_timing* out);
 
 /* The drop-in kernel launch: accumulate rows [row_begin, row_begin+num_tuples) of the
  * device columns into the caller-zeroed device table `aggregates` on `hip_stream`
- * (NULL = the context stream).  Launches on disjoint ranges compose by addition. */
+ * (NULL = the context stream).  Launches on disjoint ranges compose by addition.
+ * Large grids merge through spread tables: the context's on the context stream, a set
+ * made on a caller stream's first large launch (kept until q1_shutdown, at most 16
+ * streams; a stream under graph capture or beyond those merges directly). */
 q1_status q1_enqueue(q1_ctx* ctx, q1_placement placement, q1_layout layout,
                      const q1_launch* launch, q1_device_aggregates* aggregates,
                      const q1_columns* device, uint64_t row_begin, uint64_t num_tuples,

@@ -63,8 +63,8 @@ def test_enqueue_composes_by_addition(ctx, oracle):
 
 def test_enqueue_other_stream_and_context_stream_agree(ctx, oracle):
     """Launches on the context stream merge large grids through the context's spread
-    tables; launches on another stream merge directly.  Both add into the caller's
-    table, and back-to-back launches on each keep adding exactly."""
+    tables, launches on another stream through that stream's own.  Both add into the
+    caller's table, and back-to-back launches on each keep adding exactly."""
     import torch
     rows = 3_000_017
     ref = q1.generate_host(q1.REFERENCE, SEED + 2, rows)
@@ -88,6 +88,105 @@ def test_enqueue_other_stream_and_context_stream_agree(ctx, oracle):
                 assert (g.count, g.sum_qty, g.sum_charge, g.sum_disc_price, g.sum_base_price, g.sum_disc) == (
                     3 * w["count"], 3 * w["sum_qty"], 3 * w["sum_charge"], 3 * w["sum_disc_price"],
                     3 * w["sum_base_price"], 3 * w["sum_disc"]), (p, stream)
+    t.free()
+
+
+def _enqueue(ctx, p, cols, agg, rows, stream, launch):
+    ctx.check(q1.lib.q1_enqueue(ctx.handle, p, q1.COMPRESSED, C.byref(launch), C.c_void_p(agg.data_ptr()),
+                                C.byref(cols), 0, rows, 729999, C.c_void_p(stream)), "q1_enqueue")
+
+
+def _assert_times(agg, want, k, what):
+    got = device_aggregates_to_result(agg)
+    for g, w in zip(got, want):
+        assert (g.count, g.sum_qty, g.sum_charge, g.sum_disc_price, g.sum_base_price, g.sum_disc) == (
+            k * w["count"], k * w["sum_qty"], k * w["sum_charge"], k * w["sum_disc_price"],
+            k * w["sum_base_price"], k * w["sum_disc"]), what
+
+
+def test_enqueue_on_many_streams_and_a_recreated_stream(ctx, oracle):
+    """Large launches on 20 caller streams (the first 16 get spread tables of their own,
+    the rest merge directly), two streams adding into one table at once, and a stream
+    destroyed with its launch still queued and created again (possibly at the same
+    address, so its tables are reused after the event of their last use): every table
+    holds exactly the launches added into it."""
+    import torch
+    rows = 1_500_007
+    ref = q1.generate_host(q1.REFERENCE, SEED + 5, rows)
+    comp = q1.generate_host(q1.COMPRESSED, SEED + 5, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    cols = t.device_columns()
+    l = q1.make_launch(64, 16, 0, q1.LAUNCH_ASSUME_TPCH_DOMAIN)   # 1465 blocks
+    streams = [torch.cuda.Stream() for _ in range(20)]
+    aggs = [torch.zeros(48, dtype=torch.int64, device="cuda") for _ in streams]
+    shared = torch.zeros(48, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+    note("case 20 streams")
+    for rep in range(2):
+        for s, a in zip(streams, aggs):
+            _enqueue(ctx, q1.PER_THREAD_SHARED_MEM, cols, a, rows, s.cuda_stream, l)
+        _enqueue(ctx, q1.IN_REGISTERS, cols, shared, rows, streams[0].cuda_stream, l)
+        _enqueue(ctx, q1.IN_REGISTERS, cols, shared, rows, streams[19].cuda_stream, l)
+    torch.cuda.synchronize()
+    for i, a in enumerate(aggs):
+        _assert_times(a, want, 2, i)
+    _assert_times(shared, want, 4, "shared")
+    t.free()
+    note("case recreated stream")
+    # the HIP runtime this process already uses (the library's and torch's)
+    path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64.so" in line)
+    hip = C.CDLL(path)
+    hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    with q1.Context(0) as c2:      # a context whose stream tables are not used up yet
+        t2 = c2.upload(q1.COMPRESSED, comp, rows)
+        cols2 = t2.device_columns()
+        agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+        torch.cuda.synchronize()
+        for rep in range(3):
+            h = C.c_void_p()
+            assert hip.hipStreamCreate(C.byref(h)) == 0
+            _enqueue(c2, q1.PER_THREAD_SHARED_MEM, cols2, agg, rows, h.value, l)
+            assert hip.hipStreamDestroy(h) == 0
+        c2.synchronize()
+        torch.cuda.synchronize()
+        _assert_times(agg, want, 3, "recreated")
+        t2.free()
+
+
+def test_enqueue_under_graph_capture(ctx, oracle):
+    """q1_enqueue captured into a graph on a caller's stream (it merges directly there and
+    allocates nothing): every replay adds exactly one result, with eager launches between
+    the replays."""
+    import torch
+    rows = 1_500_007
+    ref = q1.generate_host(q1.REFERENCE, SEED + 6, rows)
+    comp = q1.generate_host(q1.COMPRESSED, SEED + 6, rows)
+    want, _ = oracle.naive(ref, 0, rows, 729999)
+    t = ctx.upload(q1.COMPRESSED, comp, rows)
+    cols = t.device_columns()
+    l = q1.make_launch(64, 16, 0, q1.LAUNCH_ASSUME_TPCH_DOMAIN)
+    agg = torch.zeros(48, dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    note("case capture")
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        _enqueue(ctx, q1.PER_THREAD_SHARED_MEM, cols, agg, rows, s.cuda_stream, l)
+    torch.cuda.synchronize()
+    _assert_times(agg, want, 0, "captured, not run")
+    k = 0
+    for rep in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        k += 1
+        _assert_times(agg, want, k, ("replay", rep))
+        _enqueue(ctx, q1.IN_REGISTERS, cols, agg, rows, s.cuda_stream, l)   # eager, between replays
+        torch.cuda.synchronize()
+        k += 1
+        _assert_times(agg, want, k, ("eager", rep))
+    del g
     t.free()
 
 

@@ -101,19 +101,61 @@ q1_status ensure_workspace(q1_ctx* ctx, uint64_t bytes) {
   return Q1_OK;
 }
 
+// The spread tables of a caller's stream for a large fast launch, made on the stream's
+// first such launch; nullptr (merge directly) while the stream is being captured into a
+// graph, where nothing may be allocated or waited for outside the capture, once
+// kMaxStreamSpreads streams have tables, or if making them fails.  A stream that has
+// tables first waits for the event of their last use.
+static StreamSpread* stream_spread(q1_ctx* ctx, void* stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing((hipStream_t)stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  auto it = ctx->stream_spreads.find(stream);
+  if (it != ctx->stream_spreads.end()) {
+    StreamSpread& ss = it->second;
+    if (!ss.usable) return nullptr;
+    if (hipStreamWaitEvent((hipStream_t)stream, ss.done, 0) != hipSuccess) {
+      ss.usable = false;
+      return nullptr;
+    }
+    return &ss;
+  }
+  if (ctx->stream_spreads.size() >= kMaxStreamSpreads) return nullptr;
+  StreamSpread ss;
+  if (hipMalloc(&ss.spread.tables, kSpreadTables * sizeof(q1_device_aggregates)) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&ss.done, hipEventDisableTiming) != hipSuccess) {
+    (void)hipFree(ss.spread.tables);
+    return nullptr;
+  }
+  return &ctx->stream_spreads.emplace(stream, ss).first->second;
+}
+
 q1_status scan_on(q1_ctx* ctx, q1_placement placement, q1_layout layout, bool fast, const ScanArgs& a,
                   const LaunchShape& shape, void* stream, bool table_outside) {
-  // Launches on the context stream run one after another, so they can share one set of
-  // spread tables; launches on other streams merge directly.
-  const bool own = stream == (void*)ctx->stream && ctx->spread.tables;
-  if (own && ctx->spread_dirty) {
-    const q1_status z = launch_zero(ctx->spread.tables, kSpreadTables, stream);
-    if (z != Q1_OK) return z;
-    ctx->spread_dirty = false;
+  // Launches on one stream run one after another, so they can share one set of spread
+  // tables: the context's for the context stream, a set of the stream's own for a
+  // caller's stream (large fast launches only; the others merge directly).
+  Spread* sp = nullptr;
+  bool* dirty = nullptr;
+  StreamSpread* other = nullptr;
+  if (stream == (void*)ctx->stream) {
+    if (ctx->spread.tables) {
+      sp = &ctx->spread;
+      dirty = &ctx->spread_dirty;
+    }
+  } else if (fast && shape.blocks >= kSpreadMinBlocks) {
+    if ((other = stream_spread(ctx, stream))) {
+      sp = &other->spread;
+      dirty = &other->dirty;
+    }
   }
-  const q1_status s = launch_scan(placement, layout, fast, a, shape, ctx->workspace, stream,
-                                  own ? &ctx->spread : nullptr, table_outside);
-  if (s != Q1_OK && own) ctx->spread_dirty = true;
+  if (sp && *dirty) {
+    const q1_status z = launch_zero(sp->tables, kSpreadTables, stream);
+    if (z != Q1_OK) return z;
+    *dirty = false;
+  }
+  const q1_status s = launch_scan(placement, layout, fast, a, shape, ctx->workspace, stream, sp, table_outside);
+  if (s != Q1_OK && sp) *dirty = true;
+  if (other && hipEventRecord(other->done, (hipStream_t)stream) != hipSuccess) other->usable = false;
   return s;
 }
 
@@ -329,6 +371,13 @@ void q1_shutdown(q1_ctx* ctx) {
   if (ctx->workspace) (void)hipFree(ctx->workspace);
   if (ctx->d_stats) (void)hipFree(ctx->d_stats);
   if (ctx->spread.tables) (void)hipFree(ctx->spread.tables);
+  for (auto& kv : ctx->stream_spreads) {
+    // the caller's stream may still run the last launch that used them
+    (void)hipEventSynchronize(kv.second.done);
+    (void)hipFree(kv.second.spread.tables);
+    (void)hipEventDestroy(kv.second.done);
+  }
+  ctx->stream_spreads.clear();
   if (ctx->h_aggr) (void)hipHostFree(ctx->h_aggr);
   if (ctx->d_aggr) (void)hipFree(ctx->d_aggr);
   for (hipEvent_t e : {ctx->ev_begin, ctx->ev_end, ctx->ev_h2d_begin, ctx->ev_h2d_end})

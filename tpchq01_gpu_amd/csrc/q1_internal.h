@@ -5,6 +5,7 @@
 
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <unordered_set>
 #include <vector>
 
@@ -31,6 +32,18 @@ constexpr int kStreamCount = 4;
 struct q1_batch_state;  // q1_batch.cpp
 namespace q1 {
 struct StreamState;     // q1_stream.cpp: the host-resident streaming pipeline
+// Spread tables of one caller stream (q1_enqueue on a stream other than the context's).
+// `done` is recorded after every launch that uses them and the next launch waits for it,
+// so a stream created at the address of a destroyed one never overlaps the old stream's
+// last use; `usable` is cleared if that bookkeeping ever fails.
+struct StreamSpread {
+  Spread spread{};
+  hipEvent_t done = nullptr;
+  bool dirty = true;   // zeroed by the first launch that uses them
+  bool usable = true;
+};
+// Caller streams that get spread tables; launches on further streams merge directly.
+constexpr size_t kMaxStreamSpreads = 16;
 // A launch shape tuned on ranges of `rows` rows ("launch_at" lines of the tuning file).
 struct SizedLaunch {
   uint64_t rows;
@@ -67,6 +80,7 @@ struct q1_ctx {
   q1::StreamState* streaming = nullptr;    // cached staging buffers and streams of q1_run_streaming
   q1::Spread spread{};                     // spread-merge tables of the context stream
   bool spread_dirty = false;               // a failed launch may have left partials in them
+  std::unordered_map<void*, q1::StreamSpread> stream_spreads;  // caller streams' spread tables
   std::unordered_set<q1_table*> tables;    // live tables (released device memory at shutdown)
   std::mutex lock;                         // serialises calls that share the context stream
 };
