@@ -44,6 +44,17 @@ def test_merge_keeps_what_was_not_swept():
     assert autotune.merge_results(old, new) == [old[0], old[2], new[0]]
 
 
+def test_batch_shape_is_the_largest_grid_within_one_percent():
+    def b(per_cu, tpt, ms, p=3):
+        return dict(layout=2, placement=p, blocks_per_cu=per_cu, tpt=tpt, kernel_ms=ms)
+    res = [b(3, 256, 1.103), b(4, 256, 1.111), b(8, 192, 1.104), b(8, 256, 1.110), b(2, 256, 1.226),
+           b(6, 512, 1.30, p=0), b(8, 512, 1.40, p=0)]
+    best = autotune.best_batch_table(res)
+    assert (best[(2, 3)]["blocks_per_cu"], best[(2, 3)]["tpt"]) == (8, 192)
+    assert (best[(2, 0)]["blocks_per_cu"], best[(2, 0)]["tpt"]) == (6, 512)   # 8 x 512 is 8 % slower
+    assert autotune.best_batch_table(res, tie=0.0)[(2, 3)]["blocks_per_cu"] == 3
+
+
 def _m(layout, placement, rows, tpb, tpt, ms):
     return dict(layout=layout, placement=placement, rows=rows, tpb=tpb, tpt=tpt, blocks=1, kernel_ms=ms,
                 gbps=rows * q1.BITS_PER_ROW[layout] / 8 / (ms * 1e-3) / 1e9)
@@ -94,7 +105,7 @@ def test_small_sweep_is_read_back(tmp_path, monkeypatch):
     assert len(res) == 2 * 2 * 2 * 2 and len(bres) == 2 * 2 * 2
     best, per_layout = autotune.write_tables(res, str(tmp_path), "test", [big, small], bres)
     best_small, _ = autotune.best_table(res, small)
-    bbest, _ = autotune.best_table(bres)
+    bbest = autotune.best_batch_table(bres)
     monkeypatch.setenv("Q1_TUNING_FILE", str(tmp_path / "gfx950.txt"))
     with q1.Context(0) as ctx:
         for (l, p), r in best.items():

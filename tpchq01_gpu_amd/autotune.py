@@ -188,6 +188,29 @@ def merge_results(old, new):
     return [r for r in old if (r["layout"], r["placement"], r["rows"]) not in swept] + list(new)
 
 
+BATCH_TIE = 0.01
+
+
+def best_batch_table(batch_results, tie=BATCH_TIE):
+    """The batched kernel's shape per (layout, placement): of the shapes within `tie` of the
+    fastest, the one with the most blocks per compute unit (then the fastest of those).
+    Grids that leave wave slots empty measured within 1 % of full ones on the batch alone,
+    but 5 % slower with other tables resident too (1.18 against 1.125 ms for 3 against 8
+    blocks per CU), so among shapes the sweep cannot tell apart the full grid is kept."""
+    fastest = {}
+    for r in batch_results:
+        k = (r["layout"], r["placement"])
+        fastest[k] = min(fastest.get(k, float("inf")), r["kernel_ms"])
+    best = {}
+    for r in batch_results:
+        k = (r["layout"], r["placement"])
+        if r["kernel_ms"] > fastest[k] * (1 + tie):
+            continue
+        if k not in best or (r["blocks_per_cu"], -r["kernel_ms"]) > (best[k]["blocks_per_cu"], -best[k]["kernel_ms"]):
+            best[k] = r
+    return best
+
+
 def best_streaming(stream_results):
     """The fastest (chunk rows, streams) per layout."""
     best = {}
@@ -204,7 +227,7 @@ def write_tables(results, out_dir: str, device: str, rows, batch_results=(), arc
     os.makedirs(out_dir, exist_ok=True)
     sizes = sizes_of(results)
     best, per_layout = best_table(results)
-    bbest, _ = best_table(batch_results)
+    bbest = best_batch_table(batch_results)
     sbest = best_streaming(stream_results)
     with open(os.path.join(out_dir, f"{arch}.txt"), "w") as f:
         swept = ", ".join(str(n) for n in sizes)
@@ -257,9 +280,9 @@ def report(table: dict) -> str:
             mark = " **best**" if per_layout[l] is r else ""
             lines.append(f"| {q1.LAYOUT_NAMES[l]} | {q1.PLACEMENT_NAMES[p]}{mark} | {r['tpb']} | {r['tpt']} | "
                          f"{r['kernel_ms']:.3f} | {r['gbps']:.0f} | {100 * frac:.1f} | `{'#' * round(40 * frac)}` |")
-    bbest, _ = best_table(table.get("batch_measurements", []))
+    bbest = best_batch_table(table.get("batch_measurements", []))
     if bbest:
-        lines += ["", "Batched kernel (256 threads per block):", "",
+        lines += ["", "Batched kernel (256 threads per block; of the shapes within 1 % of the fastest, the largest grid):", "",
                   "| layout | placement | blocks/CU | tuples/thread | requests | rows | kernel ms | GB/s | % of 8 TB/s |",
                   "|---|---|---|---|---|---|---|---|---|"]
         for (l, p), r in sorted(bbest.items()):
