@@ -1015,11 +1015,15 @@ struct AccPerThread {
 constexpr uint32_t kLocalFlushRows = 1024;
 struct AccLocal {
   static constexpr bool kPackedSmall = false;
-  // register sums of at most kLocalFlushRows rows: count, discount and quantity in the
-  // pack_small64 layout (exact up to 16384 rows), price, discount product and charge
+  // Per group: narrow sums of at most 16 rows as in AccRegisters (pack_small word, price,
+  // discount product: one 32-bit add each per row), folded every 16 rows into 64-bit
+  // register sums of at most kLocalFlushRows rows (count, discount and quantity in the
+  // pack_small64 layout, exact up to 16384 rows; price; discount product), with the charge
+  // summed in 64 bits directly.
   static constexpr uint32_t kTableWords = kNumGroups * 4;
   static constexpr uint32_t kFlushCalls = kLocalFlushRows / 16;
-  uint64_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups], nc[kNumGroups];
+  uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
+  uint64_t sm[kNumGroups], pr[kNumGroups], dp[kNumGroups], ch[kNumGroups];
   uint32_t calls;
   // The table (kTableWords words of the caller's private array) is used only through this
   // pointer, which an empty asm hides from the optimizer: the array cannot become
@@ -1031,10 +1035,13 @@ struct AccLocal {
     t = reinterpret_cast<uint64_t*>(private_table);
     __asm__ volatile("" : "+v"(t));
 #pragma unroll
-    for (int g = 0; g < kNumGroups; g++) ns[g] = np[g] = nd[g] = nc[g] = 0;
+    for (int g = 0; g < kNumGroups; g++) {
+      ns[g] = np[g] = nd[g] = 0;
+      sm[g] = pr[g] = dp[g] = ch[g] = 0;
+    }
   }
   __device__ __forceinline__ void add(const FastRow& r) {
-    const uint64_t s = pack_small64(r);
+    const uint32_t s = pack_small(r);
     const uint32_t dprod = disc_prod(r);
     const uint64_t c = fast_charge(r);
     const uint64_t pass_mask = ballot(r.pass);
@@ -1046,38 +1053,54 @@ struct AccLocal {
           ns[g] += s;
           np[g] += r.price;
           nd[g] += dprod;
-          nc[g] += c;
+          ch[g] += c;
         }
       }
     }
   }
-  // Called every 16 rows of a lane (at the same points in every lane) and after its last
-  // row (last = true); every kLocalFlushRows rows, and at the end, the register sums go into
-  // the private table.
+  // Called every 16 rows of a lane (at the same points in every lane; a lane's rows are a
+  // whole number of 16-row periods, so the last such call comes right before the final
+  // one) and after its last row (last = true).  Every call folds the narrow sums into the
+  // register sums; every kLocalFlushRows rows, and at the end unless that just happened,
+  // the register sums go into the private table.
   __device__ __forceinline__ void flush(bool last = false) {
     static_assert(kLocalFlushRows <= 16384, "pack_small64 is exact up to 16384 rows");
     if (last) {
       if (calls > 0 && calls % kFlushCalls == 0) return;  // nothing summed since the last update
-    } else if (++calls % kFlushCalls != 0) {
-      return;
+    } else {
+#pragma unroll
+      for (int g = 0; g < kNumGroups; g++) {
+        if (ballot(ns[g] != 0)) {
+          sm[g] += small_to_small64(ns[g]);
+          pr[g] += np[g];
+          dp[g] += nd[g];
+          ns[g] = np[g] = nd[g] = 0;
+        }
+      }
+      if (++calls % kFlushCalls != 0) return;
     }
-    // All words are loaded before any is added to and stored, with no branch between the
-    // loads: the lane waits for its private memory once per update, not once per group
-    // (groups without rows add zero).
+    // The table is updated in two halves of three groups: each half's words are loaded
+    // before any is added to and stored, with no branch between the loads (groups without
+    // rows add zero), so the lane waits for its private memory twice per update while
+    // holding only half the table in registers.
     const bool first = calls <= kFlushCalls;
-    uint64_t w[kTableWords];
+    constexpr int kHalf = kNumGroups / 2;
 #pragma unroll
-    for (uint32_t i = 0; i < kTableWords; i++) w[i] = first ? 0 : t[i];
+    for (int h = 0; h < kNumGroups; h += kHalf) {
+      uint64_t w[kHalf * 4];
 #pragma unroll
-    for (int g = 0; g < kNumGroups; g++) {
-      w[g * 4 + 0] += ns[g];
-      w[g * 4 + 1] += np[g];
-      w[g * 4 + 2] += nd[g];
-      w[g * 4 + 3] += nc[g];
-      ns[g] = np[g] = nd[g] = nc[g] = 0;
+      for (int i = 0; i < kHalf * 4; i++) w[i] = first ? 0 : t[h * 4 + i];
+#pragma unroll
+      for (int g = 0; g < kHalf; g++) {
+        w[g * 4 + 0] += sm[h + g];
+        w[g * 4 + 1] += pr[h + g];
+        w[g * 4 + 2] += dp[h + g];
+        w[g * 4 + 3] += ch[h + g];
+        sm[h + g] = pr[h + g] = dp[h + g] = ch[h + g] = 0;
+      }
+#pragma unroll
+      for (int i = 0; i < kHalf * 4; i++) t[h * 4 + i] = w[i];
     }
-#pragma unroll
-    for (uint32_t i = 0; i < kTableWords; i++) t[i] = w[i];
   }
   // The private table (after the last flush).
   __device__ __forceinline__ void totals(LaneTotals& o, char*) const {
