@@ -9,7 +9,7 @@
 // (38 B/row) layouts with the patterns of their kernels, and a plain 2 GiB device copy
 // (read + write rate).  Each shape is timed as 20 back-to-back launches, as bench.py
 // times the scan.  The fastest of these is the rate the scan could reach if its arithmetic
-// were free; the scan's own rate is judged against it and against the published 8 TB/s.
+// were free; the scan's own rate is compared with it and with the published 8 TB/s.
 //
 //   hbm_probe [rows]     (default 600000000; prints one JSON line)
 #include <hip/hip_runtime.h>
