@@ -134,9 +134,10 @@ def test_enqueue_on_many_streams_and_a_recreated_stream(ctx, oracle):
     _assert_times(shared, want, 4, "shared")
     t.free()
     note("case recreated stream")
-    # the HIP runtime this process already uses (the library's and torch's)
-    path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64.so" in line)
-    hip = C.CDLL(path)
+    # the HIP runtime this process already uses (the library's and torch's: one copy)
+    paths = sorted({line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64.so" in line})
+    assert len(paths) == 1, paths
+    hip = C.CDLL(paths[0])
     hip.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
     hip.hipStreamDestroy.argtypes = [C.c_void_p]
     with q1.Context(0) as c2:      # a context whose stream tables are not used up yet
