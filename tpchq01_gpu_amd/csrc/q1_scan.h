@@ -761,10 +761,11 @@ template <> struct TileOf<Q1_LAYOUT_UNCOMPRESSED> { using T = UncompressedTile; 
 template <> struct TileOf<Q1_LAYOUT_COMPRESSED> { using T = CompressedTile<false>; };
 template <> struct TileOf<Q1_LAYOUT_COMPRESSED_PUSHDOWN> { using T = CompressedTile<true>; };
 
-// The tile a scan_fast<L, P> kernel reads.  With the compressed layouts the LDS placement
-// (little register state per lane) streams 4-row tiles with non-temporal loads, four tiles
-// in flight per lane; the other placements keep TileOf<L>, whose tile sizes are also the
-// units the host counts tiles in (tile_rows).
+// The tile a scan_fast<L, P> kernel reads.  The LDS placement (little register state per
+// lane) streams small tiles with non-temporal loads, four tiles in flight per lane: 4 rows
+// for the compressed layouts, 2 for uncompressed and reference (specialisations below);
+// the register placements keep TileOf<L>, whose tile sizes are also the units the host
+// counts tiles in (tile_rows).
 template <int L, int P> struct ScanTileOf { using T = typename TileOf<L>::T; };
 
 // A group of U lane tiles t0 + m * stride (m < U, at most `count` of them, below
