@@ -97,6 +97,40 @@ def test_range_starts_around_the_128_row_alignment(ctx, oracle, layout):
     t.free()
 
 
+def tuned_shapes():
+    """Every distinct (layout, placement, threads, tuples) of the shipped tuning table's
+    "launch" and "launch_at" lines."""
+    import os
+    from tpchq01_gpu_amd import PACKAGE_DIR
+    shapes = set()
+    for line in open(os.path.join(PACKAGE_DIR, "tuning", "gfx950.txt")):
+        f = line.split()
+        if f and f[0] == "launch":
+            shapes.add(tuple(map(int, f[1:5])))
+        elif f and f[0] == "launch_at":
+            shapes.add(tuple(map(int, f[2:6])))
+    return sorted(shapes)
+
+
+def test_every_tuned_shape(ctx, oracle):
+    """Each launch shape the tuning table can hand out, for its own layout and placement,
+    on a 1M-row range starting off a tile boundary: equal to the oracle."""
+    rows, b, e = 1_000_003, 37, 1_000_003
+    ref = host_tables(rows)[q1.REFERENCE]
+    want, _ = oracle.naive(ref, b, e, 729999)
+    shapes = tuned_shapes()
+    assert len(shapes) >= 20
+    for layout in LAYOUTS:
+        t = device_table(ctx, layout, rows, 729999)
+        for l, p, tpb, tpt in shapes:
+            if l != layout:
+                continue
+            note(f"case tuned shape layout={q1.LAYOUT_NAMES[l]} placement={q1.PLACEMENT_NAMES[p]} {tpb}x{tpt}")
+            got = ctx.run(t, b, e - b, 729999, p, q1.make_launch(tpb, tpt))
+            assert as_dicts(got) == want, (l, p, tpb, tpt)
+        t.free()
+
+
 @pytest.mark.parametrize("shape", [(64, 16), (128, 256), (128, 48), (256, 4096), (256, 16384), (64, 1024)])
 def test_launch_shapes(ctx, oracle, shape):
     rows = 300_001
