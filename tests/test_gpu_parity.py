@@ -216,10 +216,12 @@ def test_device_filter_matches_oracle(ctx, oracle):
         pd = ctx.build_filter(t, cutoff)
         got = pd.download()["filter"]
         assert np.array_equal(got, oracle.precompute_filter(comp["ship_date"], rows, cutoff))
-    # sizes around the producer's 8-row lanes and 32-row words
-    for n in (1, 7, 8, 9, 31, 32, 33, 255, 257, 2049):
+    # sizes around the producer's 8-row lanes, 32-row words and 8,192-row blocks
+    big = q1.generate_host(q1.COMPRESSED, SEED + 11, 40 * 16384)
+    for n in (1, 7, 8, 9, 31, 32, 33, 255, 257, 2049, 8191, 8192, 8193, 3 * 8192 + 9, 16385,
+              40 * 16384 - 24):
         c = {k: v[: (n + 15) // 16 if k == "return_flag" else (n + 31) // 32 if k == "line_status" else n].copy()
-             for k, v in comp.items()}
+             for k, v in big.items()}
         tn = ctx.upload(q1.COMPRESSED, c, n)
         for cutoff in (729999, 729906):
             pd = ctx.build_filter(tn, cutoff)
