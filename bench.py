@@ -630,6 +630,11 @@ def main():
         log(f"bench: batch leg, {args.batch_requests} requests")
         out["batch"], batch_results = batch_leg(q1, ctx, spec,
                                                 (table, rows) if layout == q1.COMPRESSED else None)
+        # the same columns read by the batched kernel's resident-grid structure with no
+        # arithmetic (hbm_probe's resident pattern): the batch's own ceiling
+        res = probe.get("resident_gbps") if probe else None
+        out["batch"]["resident_read_ceiling_gbps"] = res
+        out["batch"]["frac_of_resident_read_ceiling"] = out["batch"]["batched_device_gbps"] / res if res else None
         log(f"bench: batch {out['batch']}")
     plan = [(float(r), float(s)) for r, s in (x.split(":") for x in args.serve_plan.split(",") if x)]
     if plan:

@@ -7,7 +7,8 @@
 // one flat buffer, and with the streaming pattern of the shared placement (4-row tiles,
 // non-temporal loads); then the columns of the uncompressed (20 B/row) and reference
 // (38 B/row) layouts with the patterns of their kernels, and a plain 2 GiB device copy
-// (read + write rate).  Each shape is timed as 20 back-to-back launches, as bench.py
+// (read + write rate), and the batched kernel's resident-grid structure over the compressed
+// columns.  Each shape is timed as 20 back-to-back launches, as bench.py
 // times the scan.  The fastest of these is the rate the scan could reach if its arithmetic
 // were free; the scan's own rate is compared with it and with the published 8 TB/s.
 //
@@ -82,6 +83,42 @@ __global__ __launch_bounds__(B) void read_columns_nt(Columns c, uint64_t tiles4,
     const uint32_t x = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(c.tax) + t);
     const u32x4_t p = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(c.price) + t);
     acc += s.x ^ s.y ^ q ^ d ^ x ^ p.x ^ p.y ^ p.z ^ p.w ^ rf[t] ^ (ls[t >> 1] >> (4 * (t & 1)));
+  }
+  if (acc == kSentinel) sink[0] = acc;
+}
+
+// The batched kernel's structure with the same loads and no arithmetic: a resident grid
+// works through a list of entries of B lanes x tiles_per_entry 4-row tiles, block b first
+// running entry b and then claiming entries gridDim.x, gridDim.x + 1, ... from `counter`
+// (0 at launch) in the order blocks finish.  Every block exits once the claims pass the
+// last entry.
+template <int B>
+__global__ __launch_bounds__(B) void read_columns_resident(Columns c, uint64_t tiles4, uint32_t tiles_per_entry,
+                                                           uint32_t* counter, uint32_t* sink) {
+  __shared__ uint32_t next;
+  const uint64_t per_entry = (uint64_t)B * tiles_per_entry;
+  const uint64_t entries = (tiles4 + per_entry - 1) / per_entry;
+  const uint8_t* rf = reinterpret_cast<const uint8_t*>(c.return_flag);
+  const uint8_t* ls = reinterpret_cast<const uint8_t*>(c.line_status);
+  uint32_t acc = 0;
+  uint64_t v = blockIdx.x;
+  while (v < entries) {
+    uint32_t claim = 0;
+    if (threadIdx.x == 0) claim = atomicInc(counter, 0xFFFFFFFFu);
+    const uint64_t begin = v * per_entry;
+    const uint64_t end = begin + per_entry < tiles4 ? begin + per_entry : tiles4;
+    for (uint64_t t = begin + threadIdx.x; t < end; t += B) {
+      const u32x2_t s = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t*>(c.ship) + t);
+      const uint32_t q = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(c.quantity) + t);
+      const uint32_t d = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(c.discount) + t);
+      const uint32_t x = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(c.tax) + t);
+      const u32x4_t p = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(c.price) + t);
+      acc += s.x ^ s.y ^ q ^ d ^ x ^ p.x ^ p.y ^ p.z ^ p.w ^ rf[t] ^ (ls[t >> 1] >> (4 * (t & 1)));
+    }
+    if (threadIdx.x == 0) next = gridDim.x + claim;
+    __syncthreads();
+    v = (uint32_t)__builtin_amdgcn_readfirstlane((int)next);
+    __syncthreads();  // every lane has read `next` before thread 0 writes it again
   }
   if (acc == kSentinel) sink[0] = acc;
 }
@@ -281,7 +318,21 @@ int main(int argc, char** argv) {
     std::printf("%s{\"threads_per_block\": %d, \"tuples_per_thread\": %d, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "",
                 b, stream_shapes[s][1], ms, bytes / ms / 1e6);
   }
-  std::printf("], \"flat\": [");
+  // the batched kernel's tuned structure: 8 resident blocks per compute unit x 256 lanes,
+  // entries of 192 rows per lane claimed one at a time
+  int cus = 0;
+  CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  uint32_t* counter = nullptr;
+  CHECK(hipMalloc(&counter, 4));
+  const uint32_t res_blocks = 8u * (uint32_t)cus, res_tiles = 192 / 4;
+  const double res_ms = back_to_back_ms([&] {
+    CHECK(hipMemsetAsync(counter, 0, 4));
+    read_columns_resident<256><<<dim3(res_blocks), dim3(256)>>>(c, tiles4, res_tiles, counter, sink);
+  }, reps);
+  std::printf("], \"resident_streaming\": {\"blocks\": %u, \"threads_per_block\": 256, \"rows_per_lane\": 192, "
+              "\"ms\": %.4f, \"gbps\": %.1f}", res_blocks, res_ms, bytes / res_ms / 1e6);
+  CHECK(hipFree(counter));
+  std::printf(", \"flat\": [");
   const uint64_t n16 = bytes / 16;
   const int flat[4][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 64}};
   for (int s = 0; s < 4; s++) {
@@ -340,6 +391,7 @@ int main(int argc, char** argv) {
   std::printf(fmt, 2, "true", r2, ref_bytes / r2 / 1e6);
   std::printf("], \"copy\": {\"bytes\": %llu, \"ms\": %.4f, \"gbps\": %.1f}", (unsigned long long)copy_bytes,
               copy_ms, 2.0 * copy_bytes / copy_ms / 1e6);
+  std::printf(", \"resident_gbps\": %.1f", bytes / res_ms / 1e6);
   std::printf(", \"best_columns_gbps\": %.1f, \"best_streaming_gbps\": %.1f, \"best_flat_gbps\": %.1f, "
               "\"best_uncompressed_gbps\": %.1f, \"best_reference_gbps\": %.1f, \"copy_gbps\": %.1f}\n",
               bytes / best_cols / 1e6, bytes / best_stream / 1e6, n16 * 16 / best_flat / 1e6,
