@@ -13,7 +13,12 @@ favour fewer rows per lane, whose last round of blocks leaves less of the chip i
 Batch shapes: for each batch placement (the register table and the LDS table) it sweeps
 the batched kernel's grid (blocks per compute unit) x rows per lane on the 256-request
 batch workload (uneven log-uniform requests, their own cutoffs), timing q1_run_batch's
-kernel.
+kernel; of the shapes within 1 % of the fastest it keeps the one with the most blocks per
+compute unit.
+
+Streaming (--streaming): the host-resident pipeline's rows per chunk x streams over 600M
+compressed rows in pinned host memory, end-to-end time per query (the reference's
+--tuples-per-kernel-launch and --streams).
 
 It writes
   tuning/<arch>.txt    "launch <layout> <placement> <tpb> <tpt>" and "best <layout> <placement>"
@@ -21,14 +26,16 @@ It writes
                        <placement> <tpb> <tpt>" lines (every row count, when several were
                        swept; a launch takes the one nearest its own row count) and
                        "batch <layout> <placement> <blocks per CU> <rows per lane>"
-                       lines, read by q1_init (or Q1_TUNING_FILE)
+                       and "streaming <layout> <chunk rows> <streams>" lines, read by
+                       q1_init (or Q1_TUNING_FILE)
   tuning/<arch>.json   every measurement
-and `--report` prints the optimum per layout and placement as a table
-(tuning/<arch>_report.md), the place of the reference's per-configuration optimum list and
-its plots (scripts/benchmark_with_optimal_params.py:39-55, scripts/plots.R).
+and `--report` prints the optimum per row count, layout and placement as tables
+(tuning/<arch>_report.md), the place of the reference's per-configuration optimum list
+(scripts/benchmark_with_optimal_params.py:39-55); tpchq01_gpu_amd.plots draws the sweeps
+(scripts/plots.R).
 
 Usage: python -m tpchq01_gpu_amd.autotune [--rows N[,N...]] [--layouts compressed,...] [--batch]
-       [--merge] [--report]
+       [--streaming] [--merge] [--report]
 """
 from __future__ import annotations
 
