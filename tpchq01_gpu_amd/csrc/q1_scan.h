@@ -875,24 +875,29 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 // Q1_PLACE_IN_REGISTERS: narrow per-tile accumulators in registers; a group's block of
-// adds is skipped by the whole wavefront when no lane holds a row of that group.
+// adds is skipped by the whole wavefront when no lane holds a row of that group.  Price
+// and price x discount share one 64-bit word (price in the low half, below 16 * 2^24 =
+// 2^28 over a tile; the product in the high half, below 16 * 15 * 2^24 < 2^32), so a row
+// costs three adds per group: that word, the packed count|discount|quantity and the charge.
 struct AccRegisters {
   static constexpr bool kPackedSmall = false;
-  uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
+  uint32_t ns[kNumGroups];
+  uint64_t npd[kNumGroups];
   uint64_t ch[kNumGroups], pr[kNumGroups], dp[kNumGroups];
   uint32_t cnt[kNumGroups], dsc[kNumGroups], qty[kNumGroups];
 
   __device__ __forceinline__ void init(char*) {
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      ns[g] = np[g] = nd[g] = 0;
+      ns[g] = 0;
+      npd[g] = 0;
       ch[g] = pr[g] = dp[g] = 0;
       cnt[g] = dsc[g] = qty[g] = 0;
     }
   }
   __device__ __forceinline__ void add(const FastRow& r) {
     const uint32_t s = pack_small(r);
-    const uint32_t dprod = disc_prod(r);
+    const uint64_t pd = (uint64_t)disc_prod(r) << 32 | r.price;
     const uint64_t c = fast_charge(r);
     const uint64_t pass_mask = ballot(r.pass);
 #pragma unroll
@@ -904,8 +909,7 @@ struct AccRegisters {
         __asm__ volatile("");
         if (mine) {
           ns[g] += s;
-          np[g] += r.price;
-          nd[g] += dprod;
+          npd[g] += pd;
           ch[g] += c;
         }
       }
@@ -919,9 +923,10 @@ struct AccRegisters {
         cnt[g] += ns[g] & 31u;
         dsc[g] += (ns[g] >> 5) & 255u;
         qty[g] += ns[g] >> 13;
-        pr[g] += np[g];
-        dp[g] += nd[g];
-        ns[g] = np[g] = nd[g] = 0;
+        pr[g] += (uint32_t)npd[g];
+        dp[g] += npd[g] >> 32;
+        ns[g] = 0;
+        npd[g] = 0;
       }
     }
   }
@@ -1016,14 +1021,15 @@ struct AccPerThread {
 constexpr uint32_t kLocalFlushRows = 1024;
 struct AccLocal {
   static constexpr bool kPackedSmall = false;
-  // Per group: narrow sums of at most 16 rows as in AccRegisters (pack_small word, price,
-  // discount product: one 32-bit add each per row), folded every 16 rows into 64-bit
+  // Per group: narrow sums of at most 16 rows as in AccRegisters (pack_small word; price
+  // and discount product in one 64-bit word), folded every 16 rows into 64-bit
   // register sums of at most kLocalFlushRows rows (count, discount and quantity in the
   // pack_small64 layout, exact up to 16384 rows; price; discount product), with the charge
   // summed in 64 bits directly.
   static constexpr uint32_t kTableWords = kNumGroups * 4;
   static constexpr uint32_t kFlushCalls = kLocalFlushRows / 16;
-  uint32_t ns[kNumGroups], np[kNumGroups], nd[kNumGroups];
+  uint32_t ns[kNumGroups];
+  uint64_t npd[kNumGroups];
   uint64_t sm[kNumGroups], pr[kNumGroups], dp[kNumGroups], ch[kNumGroups];
   uint32_t calls;
   // The table (kTableWords words of the caller's private array) is used only through this
@@ -1037,13 +1043,14 @@ struct AccLocal {
     __asm__ volatile("" : "+v"(t));
 #pragma unroll
     for (int g = 0; g < kNumGroups; g++) {
-      ns[g] = np[g] = nd[g] = 0;
+      ns[g] = 0;
+      npd[g] = 0;
       sm[g] = pr[g] = dp[g] = ch[g] = 0;
     }
   }
   __device__ __forceinline__ void add(const FastRow& r) {
     const uint32_t s = pack_small(r);
-    const uint32_t dprod = disc_prod(r);
+    const uint64_t pd = (uint64_t)disc_prod(r) << 32 | r.price;
     const uint64_t c = fast_charge(r);
     const uint64_t pass_mask = ballot(r.pass);
 #pragma unroll
@@ -1052,8 +1059,7 @@ struct AccLocal {
         __asm__ volatile("");
         if (r.pass && r.gid == (uint32_t)g) {
           ns[g] += s;
-          np[g] += r.price;
-          nd[g] += dprod;
+          npd[g] += pd;
           ch[g] += c;
         }
       }
@@ -1073,9 +1079,10 @@ struct AccLocal {
       for (int g = 0; g < kNumGroups; g++) {
         if (ballot(ns[g] != 0)) {
           sm[g] += small_to_small64(ns[g]);
-          pr[g] += np[g];
-          dp[g] += nd[g];
-          ns[g] = np[g] = nd[g] = 0;
+          pr[g] += (uint32_t)npd[g];
+          dp[g] += npd[g] >> 32;
+          ns[g] = 0;
+          npd[g] = 0;
         }
       }
       if (++calls % kFlushCalls != 0) return;
