@@ -8,9 +8,9 @@ through the C ABI:
     q1_enqueue with and without the domain flag, and the CPU/GPU split at the reference's
     fraction 0.5, vs KernelNaive over the same seeded rows regenerated on the host in
     chunks;
-  * 4,500,000,017 device-generated compressed rows: counts, offsets and batch requests past
-    2^32, against chunked runs and against KernelNaive on the ranges around row 2^32 and
-    at the end;
+  * 4,500,000,017 device-generated rows in the compressed, uncompressed and reference
+    layouts (42, 90 and 171 GB): counts, offsets and batch requests past 2^32, against
+    chunked runs and against KernelNaive on the ranges around row 2^32 and at the end;
   * the 256-request batch (uneven log-uniform sizes, own cutoffs): every request vs
     KernelNaive on its own rows;
   * the serving mix (8 resident tables of 1M..600M rows, Poisson arrivals at 2000 req/s):
@@ -154,55 +154,60 @@ def test_scan_600M_compressed_rows(ctx, oracle):
         t.free()
 
 
-def test_scan_beyond_2_32_rows(ctx, oracle):
-    """4,500,000,017 device-generated compressed rows (42 GB of columns): row counts and
-    offsets past 2^32 (the reference counts rows in 32 bits, this ABI in 64).
+@pytest.mark.parametrize("layout", [q1.COMPRESSED, q1.UNCOMPRESSED, q1.REFERENCE],
+                         ids=["compressed_42GB", "uncompressed_90GB", "reference_171GB"])
+def test_scan_beyond_2_32_rows(ctx, oracle, layout):
+    """4,500,000,017 device-generated rows in each layout (42, 90 and 171 GB of columns): row
+    counts and offsets past 2^32 (the reference counts rows in 32 bits, this ABI in 64), and
+    byte offsets of the 8-byte columns past 2^35.
       * at the generator's last ship date (1998-12-01) every row passes: in every
         placement the groups' counts add up to the row count;
-      * at 1998-09-02 every placement, the pushdown table and a batch request over the whole
-        range equal the exact sum of runs over 2^30-row chunks;
+      * at 1998-09-02 every placement, the pushdown table (compressed) and a batch request
+        over the whole range equal the exact sum of runs over 2^30-row chunks;
       * 2^25 rows straddling row 2^32, the 12 rows around it and the ragged last ~1M rows
         equal KernelNaive over the same rows regenerated on the host, in every placement
         and as a batch request."""
     rows, seed, cutoff, every_row = 4_500_000_017, workloads.SCAN_SEED, 729999, 730089
-    t = ctx.generate(q1.COMPRESSED, seed, rows)
+    name = q1.LAYOUT_NAMES[layout]
+    t = ctx.generate(layout, seed, rows)
     try:
         for p in PLACEMENTS + [q1.AUTO]:
-            note(f"case 4.5G rows all pass placement={p}")
+            note(f"case 4.5G {name} rows all pass placement={p}")
             got = as_dicts(ctx.run(t, 0, rows, every_row, p))
             assert sum(g["count"] for g in got) == rows, p
-        note("case 4.5G rows in 2^30-row chunks")
+        note(f"case 4.5G {name} rows in 2^30-row chunks")
         chunked = [dict.fromkeys(FIELDS, 0) for _ in range(6)]
         for b in range(0, rows, 1 << 30):
             chunked = add_results(chunked, as_dicts(ctx.run(t, b, min(1 << 30, rows - b), cutoff, q1.AUTO)))
         for p in PLACEMENTS + [q1.AUTO]:
-            note(f"case 4.5G rows whole range placement={p}")
+            note(f"case 4.5G {name} rows whole range placement={p}")
             assert as_dicts(ctx.run(t, 0, rows, cutoff, p)) == chunked, p
-        note("case 4.5G rows pushdown")
-        pd = ctx.build_filter(t, cutoff)
-        try:
-            assert as_dicts(ctx.run(pd, 0, rows, 0, q1.AUTO)) == chunked
-        finally:
-            pd.free()
+        if layout == q1.COMPRESSED:
+            note(f"case 4.5G {name} rows pushdown")
+            pd = ctx.build_filter(t, cutoff)
+            try:
+                assert as_dicts(ctx.run(pd, 0, rows, 0, q1.AUTO)) == chunked
+            finally:
+                pd.free()
 
         straddle = (2**32 - 2**24, 2**25)
         tail_begin = (rows - 1_000_000) // 32 * 32
         ranges = [straddle, (tail_begin, rows - tail_begin)]
         for begin, count in ranges:
-            note(f"case 4.5G rows oracle over [{begin}, {begin + count})")
+            note(f"case 4.5G {name} rows oracle over [{begin}, {begin + count})")
             want = oracle_seeded(oracle, seed, begin, count, [cutoff])[cutoff]
             for p in PLACEMENTS + [q1.AUTO]:
                 assert as_dicts(ctx.run(t, begin, count, cutoff, p)) == want, (begin, p)
             if begin == straddle[0]:
                 want_straddle = want
-        note("case 4.5G rows: the 12 rows around row 2^32")
+        note(f"case 4.5G {name} rows: the 12 rows around row 2^32")
         ref = q1.generate_host(q1.REFERENCE, seed, 64, 2**32 - 32, 1)
         want, unexpected = oracle.naive_parallel(ref, 27, 39, every_row, 1)
         assert unexpected == 0 and sum(g["count"] for g in want) == 12
         for p in PLACEMENTS + [q1.AUTO]:
             assert as_dicts(ctx.run(t, 2**32 - 5, 12, every_row, p)) == want, p
 
-        note("case 4.5G rows batch")
+        note(f"case 4.5G {name} rows batch")
         reqs = (q1.Request * 2)(q1.Request(t.handle, 0, rows, cutoff, 0),
                                 q1.Request(t.handle, straddle[0], straddle[1], cutoff, 0))
         outs = (q1.Result * 2)()
