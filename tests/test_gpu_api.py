@@ -344,8 +344,8 @@ def test_pinned_columns_stream_and_split(ctx):
 
 def test_batch_small_grids_span_many_requests(ctx):
     """Grids of 1, 2 and 3 blocks make every block work through many requests' blocks,
-    claimed one after another (256 requests of about 65 tiles, aligned and unaligned
-    starts): every request must still be counted whole, as with the default grid."""
+    one after another (256 requests of about 65 tiles, aligned and unaligned starts):
+    every request must still be counted whole, as with the default grid."""
     t = ctx.generate(q1.COMPRESSED, SEED + 31, 70_000)
     for rows, begins in ((1040, [0] * 256), (1040, [16 * (i % 7) for i in range(256)]),
                          (1037, [3 + (i % 5) for i in range(256)])):

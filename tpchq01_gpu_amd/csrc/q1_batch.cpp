@@ -1,8 +1,10 @@
 // q1_run_batch: many filter-aggregate requests (uneven row counts, their own cutoffs) in
 // one persistent launch, captured once into a hipGraph and replayed:
-//   H2D copy of the request descriptors (pinned staging; it also resets the kernel's claim
-//   counter) -> zero the per-request tables (a fill kernel, never a memset node) -> the
-//   batched kernel -> D2H copy of the tables.
+//   H2D copy of the request descriptors (pinned staging) -> zero the per-request tables (a
+//   fill kernel, never a memset node) -> the batched kernel -> the fold of the requests'
+//   spread tables into their results -> D2H copy of the results.
+// The spread tables (64 per request for the first kBatchSpreadRequests requests) are
+// zeroed once when allocated; every fold clears the tables it read.
 // The graph is rebuilt only when the capacity bucket, layout, placement or grid changes;
 // each call only rewrites the pinned descriptor buffer before the replay.
 #include <algorithm>
@@ -20,6 +22,8 @@ struct q1_batch_state {
   q1::BatchHeader* d_buf = nullptr;
   q1_device_aggregates* h_out = nullptr;  // pinned
   q1_device_aggregates* d_out = nullptr;
+  q1_device_aggregates* d_spread = nullptr;  // spread_requests x kSpreadTables tables
+  uint32_t spread_requests = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -34,6 +38,7 @@ void batch_destroy(q1_batch_state* b) {
   if (b->h_out) (void)hipHostFree(b->h_out);
   if (b->d_buf) (void)hipFree(b->d_buf);
   if (b->d_out) (void)hipFree(b->d_out);
+  if (b->d_spread) (void)hipFree(b->d_spread);
   delete b;
 }
 
@@ -89,10 +94,15 @@ static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placeme
   b->capacity = cap;
   b->threads = threads;
   b->blocks = blocks;
+  b->spread_requests = std::min(cap, kBatchSpreadRequests);
+  const uint64_t spread_bytes = (uint64_t)b->spread_requests * kSpreadTables * sizeof(q1_device_aggregates);
   bool ok = hipHostMalloc(&b->h_buf, buf_bytes(cap), hipHostMallocDefault) == hipSuccess &&
             hipHostMalloc(&b->h_out, cap * sizeof(q1_device_aggregates), hipHostMallocDefault) == hipSuccess &&
             hipMalloc(&b->d_buf, buf_bytes(cap)) == hipSuccess &&
-            hipMalloc(&b->d_out, cap * sizeof(q1_device_aggregates)) == hipSuccess;
+            hipMalloc(&b->d_out, cap * sizeof(q1_device_aggregates)) == hipSuccess &&
+            hipMalloc(&b->d_spread, spread_bytes) == hipSuccess &&
+            hipMemsetAsync(b->d_spread, 0, spread_bytes, ctx->stream) == hipSuccess &&
+            hipStreamSynchronize(ctx->stream) == hipSuccess;  // before the capture, not inside it
   if (!ok) {
     batch_destroy(b);
     return fail(ctx, Q1_ENOMEM, "batch buffers could not be allocated");
@@ -105,7 +115,8 @@ static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placeme
   if (st == Q1_OK && hipMemcpyAsync(b->d_buf, b->h_buf, buf_bytes(cap), hipMemcpyHostToDevice, s) != hipSuccess)
     st = Q1_EDEVICE;
   if (st == Q1_OK) st = launch_zero(b->d_out, cap, s);
-  if (st == Q1_OK) st = launch_batch(layout, placement, b->d_buf, b->d_out, blocks, threads, s);
+  if (st == Q1_OK)
+    st = launch_batch(layout, placement, b->d_buf, b->d_out, b->d_spread, b->spread_requests, blocks, threads, s);
   if (st == Q1_OK &&
       hipMemcpyAsync(b->h_out, b->d_out, cap * sizeof(q1_device_aggregates), hipMemcpyDeviceToHost, s) != hipSuccess)
     st = Q1_EDEVICE;
@@ -170,8 +181,7 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     }
     const uint32_t blocks =
         launch && launch->num_blocks ? launch->num_blocks : batch_default_blocks(ctx, layout, p);
-    // The grid's work-items (blocks x 256) must stay below 2^32, and the kernel's claims
-    // count in 32 bits from gridDim.x up to the number of blocks of rows.
+    // The grid's work-items (blocks x 256) must stay below 2^32.
     if (blocks >= (1u << 24)) return fail(ctx, Q1_EINVAL, "batched num_blocks must be below 2^24");
     uint32_t cap = 16;
     while (cap < n) cap <<= 1;
@@ -188,12 +198,17 @@ extern "C" q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t 
     b->h_buf->n = n;
     b->h_buf->total_blocks = entries;
     b->h_buf->tiles_per_thread = lane_rows / layout_tile_rows(layout);
-    b->h_buf->claimed = 0;
+    b->h_buf->spread_requests = b->spread_requests;
     const auto w0 = std::chrono::steady_clock::now();
     if (hipEventRecord(ctx->ev_begin, ctx->stream) != hipSuccess ||
         hipGraphLaunch(b->exec, ctx->stream) != hipSuccess || hipEventRecord(ctx->ev_end, ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      // The spread tables may hold partial sums: the next call rebuilds the state.
+      (void)hipStreamSynchronize(ctx->stream);
+      batch_destroy(ctx->batch);
+      ctx->batch = nullptr;
       return fail(ctx, Q1_EDEVICE, "batch graph launch failed");
+    }
     for (uint32_t i = 0; i < n; i++) q1_aggregates_to_result(&b->h_out[i], &outs[i]);
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ctx->ev_begin, ctx->ev_end);

@@ -88,8 +88,11 @@ q1_status launch_build_filter(const uint16_t* ship, uint64_t rows, int32_t cutof
 // buffer starts with a BatchHeader followed by `n` BatchDesc records.  Request r is cut
 // into the blocks launch_scan would run over its rows alone (256 lanes x
 // tiles_per_thread tiles each); the blocks of all requests, in request order, form one
-// list of total_blocks entries.  A resident grid works through the list, claiming
-// entries from `claimed` (0 at launch), and merges each entry into outs[r].
+// list of total_blocks entries.  A resident grid of G blocks works through the list with a
+// fixed stride (block b runs entries b, b + G, b + 2G, ...).  Entry v of request r merges
+// into spread table v mod 64 of the request (spread + 64 r) when r < spread_requests, and
+// into outs[r] otherwise; after the kernel, fold_spread (one block per request with spread
+// tables) adds each request's 64 tables into outs[r] and clears them.
 struct BatchDesc {
   q1_columns cols;
   uint64_t row_begin, row_end;
@@ -101,18 +104,23 @@ struct BatchHeader {
   uint64_t n;                 // requests
   uint64_t total_blocks;      // entries of the block list (below 2^31)
   uint32_t tiles_per_thread;  // tiles of layout_tile_rows rows per lane and entry
-  uint32_t claimed;           // entries claimed beyond the first gridDim.x; 0 at launch
+  uint32_t spread_requests;   // requests 0 .. spread_requests-1 merge through spread tables
   uint64_t pad[5];
 };
 constexpr uint32_t kBatchThreads = 256;
+// Requests of a batch that get spread tables: 64 x 384 B each, at most 24 MiB.
+constexpr uint32_t kBatchSpreadRequests = 1024;
 uint32_t layout_tile_rows(q1_layout layout);
 // The placement the batched kernel runs for a requested one (shared or in registers),
 // the largest rows per lane it accepts for it, and how many of its blocks fit on a CU.
 q1_placement batch_placement(q1_placement p);
 uint32_t batch_max_lane_rows(q1_placement p);
 uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p);
+// The batched kernel over the list in dev_buffer, then, with `spread` (spread_requests x
+// kSpreadTables tables, all zero), the fold of the first spread_requests requests' tables.
 q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* dev_buffer,
-                       q1_device_aggregates* outs, uint32_t blocks, uint32_t threads, void* stream);
+                       q1_device_aggregates* outs, q1_device_aggregates* spread, uint32_t spread_requests,
+                       uint32_t blocks, uint32_t threads, void* stream);
 // The host function launch_batch launches for (layout, placement), e.g. to find its node in
 // a captured graph.
 const void* batch_kernel_function(q1_layout layout, q1_placement placement);

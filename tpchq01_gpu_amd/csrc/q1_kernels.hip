@@ -24,10 +24,12 @@ __global__ void zero_words(uint64_t* p, uint64_t n) {
 // Each wave clears only the words it read, after its sums consumed them.
 static_assert(kSpreadTables == 64, "the fold gives each table one lane of a wavefront");
 static_assert(sizeof(q1_device_aggregates) == 48 * 8, "8 fields x 6 groups");
+// Block b folds tables[64 b .. 64 b + 63] into out[b] (one block for a scan's tables, one
+// per request for a batch's).
 __global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables, q1_device_aggregates* out) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint64_t* mine = reinterpret_cast<uint64_t*>(tables + lane) + wave * 12;
-  uint64_t* dst = reinterpret_cast<uint64_t*>(out) + wave * 12;
+  uint64_t* mine = reinterpret_cast<uint64_t*>(tables + (uint64_t)blockIdx.x * kSpreadTables + lane) + wave * 12;
+  uint64_t* dst = reinterpret_cast<uint64_t*>(out + blockIdx.x) + wave * 12;
   uint64_t v[12];
 #pragma unroll
   for (int i = 0; i < 12; i++) v[i] = mine[i];
@@ -403,14 +405,20 @@ uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p) {
 }
 
 q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* buf, q1_device_aggregates* outs,
-                       uint32_t blocks, uint32_t threads, void* stream) {
+                       q1_device_aggregates* spread, uint32_t spread_requests, uint32_t blocks, uint32_t threads,
+                       void* stream) {
   placement = batch_placement(placement);
-  if (threads != (uint32_t)kBatchBlock || blocks == 0) return Q1_EINVAL;
+  if (threads != (uint32_t)kBatchBlock || blocks == 0 || (spread_requests && !spread)) return Q1_EINVAL;
   const BatchDesc* descs = reinterpret_cast<const BatchDesc*>(buf + 1);
-  uint32_t* claimed = &buf->claimed;
-  void* args[] = {(void*)&buf, (void*)&descs, (void*)&claimed, (void*)&outs};
-  return hip_status(hipLaunchKernel(batch_kernel(layout, placement), dim3(blocks), dim3(threads), args,
-                                    batch_lds(layout, placement), (hipStream_t)stream));
+  void* args[] = {(void*)&buf, (void*)&descs, (void*)&outs, (void*)&spread};
+  const hipStream_t s = (hipStream_t)stream;
+  q1_status st = hip_status(
+      hipLaunchKernel(batch_kernel(layout, placement), dim3(blocks), dim3(threads), args, batch_lds(layout, placement), s));
+  if (st == Q1_OK && spread_requests) {
+    hipLaunchKernelGGL(fold_spread, dim3(spread_requests), dim3(256), 0, s, spread, outs);
+    st = hip_status(hipGetLastError());
+  }
+  return st;
 }
 
 const void* batch_kernel_function(q1_layout layout, q1_placement placement) {

@@ -73,6 +73,8 @@ struct Slot {
   q1::BatchHeader* d_buf = nullptr;
   q1_device_aggregates* h_out = nullptr;
   q1_device_aggregates* d_out = nullptr;
+  q1_device_aggregates* d_spread = nullptr;  // the batch's per-request spread tables
+  uint32_t spread_requests = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t kernel_node = nullptr;  // the batched kernel's node (its grid is set per batch)
@@ -118,6 +120,9 @@ void free_slot_buffers(Slot& s) {
   if (s.h_out) (void)hipHostFree(s.h_out);
   if (s.d_buf) (void)hipFree(s.d_buf);
   if (s.d_out) (void)hipFree(s.d_out);
+  if (s.d_spread) (void)hipFree(s.d_spread);
+  s.d_spread = nullptr;
+  s.spread_requests = 0;
   s.exec = nullptr;
   s.graph = nullptr;
   s.kernel_node = nullptr;
@@ -132,11 +137,18 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
   free_slot_buffers(s);
   const uint64_t bb = sizeof(q1::BatchHeader) + (uint64_t)cap * sizeof(q1::BatchDesc);
   const uint64_t ob = (uint64_t)cap * sizeof(q1_device_aggregates);
+  s.spread_requests = std::min(cap, q1::kBatchSpreadRequests);
+  const uint64_t sb = (uint64_t)s.spread_requests * q1::kSpreadTables * sizeof(q1_device_aggregates);
   if (hipHostMalloc(&s.h_buf, bb, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc(&s.h_out, ob, hipHostMallocDefault) != hipSuccess || hipMalloc(&s.d_buf, bb) != hipSuccess ||
-      hipMalloc(&s.d_out, ob) != hipSuccess) {
+      hipMalloc(&s.d_out, ob) != hipSuccess || hipMalloc(&s.d_spread, sb) != hipSuccess) {
     free_slot_buffers(s);
     return Q1_ENOMEM;
+  }
+  // zeroed once, before the capture (every fold clears the tables it read)
+  if (hipMemsetAsync(s.d_spread, 0, sb, s.stream) != hipSuccess || hipStreamSynchronize(s.stream) != hipSuccess) {
+    free_slot_buffers(s);
+    return Q1_EDEVICE;
   }
   std::memset(s.h_buf, 0, bb);
   const uint32_t blocks = q1::batch_default_blocks(sv->ctx, layout, sv->placement);
@@ -145,7 +157,8 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
   if (st == Q1_OK && hipMemcpyAsync(s.d_buf, s.h_buf, bb, hipMemcpyHostToDevice, s.stream) != hipSuccess) st = Q1_EDEVICE;
   if (st == Q1_OK) st = q1::launch_zero(s.d_out, cap, s.stream);
   if (st == Q1_OK)
-    st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, blocks, q1::kBatchThreads, s.stream);
+    st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, s.d_spread, s.spread_requests, blocks,
+                          q1::kBatchThreads, s.stream);
   if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
     st = Q1_EDEVICE;
   hipGraph_t g = nullptr;
@@ -264,19 +277,20 @@ q1_status launch_slot(q1_server* sv, Slot& s) {
   s.h_buf->n = n;
   s.h_buf->total_blocks = entries;
   s.h_buf->tiles_per_thread = lane_rows / q1::layout_tile_rows(t0->layout);
-  s.h_buf->claimed = 0;
+  s.h_buf->spread_requests = s.spread_requests;
   if (s.kernel_node) {
     // The tuned persistent grid, or one block per entry for a batch with fewer entries
-    // (block b then runs entry b and claims nothing).
+    // (block b then runs entry b only).
     hipKernelNodeParams kp = s.kernel_params;
     kp.gridDim = dim3((uint32_t)std::min<uint64_t>(std::max<uint64_t>(entries, 1), s.grid));
     if (hipGraphExecKernelNodeSetParams(s.exec, s.kernel_node, &kp) != hipSuccess) s.kernel_node = nullptr;
   }
-  if (hipGraphLaunch(s.exec, s.stream) != hipSuccess) return Q1_EDEVICE;
-  if (hipEventRecord(s.done, s.stream) != hipSuccess) {
+  if (hipGraphLaunch(s.exec, s.stream) != hipSuccess || hipEventRecord(s.done, s.stream) != hipSuccess) {
     // The graph may still be reading the slot's pinned buffers: let it finish before the
-    // caller frees the slot for the next batch.
+    // caller frees the slot for the next batch, and rebuild the slot (its spread tables
+    // may hold partial sums).
     (void)hipStreamSynchronize(s.stream);
+    free_slot_buffers(s);
     return Q1_EDEVICE;
   }
   return Q1_OK;
