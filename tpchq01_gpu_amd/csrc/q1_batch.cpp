@@ -1,10 +1,13 @@
 // q1_run_batch: many filter-aggregate requests (uneven row counts, their own cutoffs) in
 // one persistent launch, captured once into a hipGraph and replayed:
-//   H2D copy of the request descriptors (pinned staging) -> zero the per-request tables (a
-//   fill kernel, never a memset node) -> the batched kernel -> the fold of the requests'
-//   spread tables into their results -> D2H copy of the results.
-// The spread tables (64 per request for the first kBatchSpreadRequests requests) are
-// zeroed once when allocated; every fold clears the tables it read.
+//   H2D copy of the request descriptors (pinned staging) -> the batched kernel -> the fold
+//   of each request's spread tables, which stores the results straight into the pinned
+//   host results (over PCIe).
+// A batch of more than kBatchSpreadRequests requests gives spread tables to the first
+// kBatchSpreadRequests only, so its graph is: H2D copy -> zero the device results (a fill
+// kernel, never a memset node) -> the batched kernel -> the fold, adding into the device
+// results -> D2H copy of the results.
+// The spread tables are zeroed once when allocated; every fold clears the tables it read.
 // The graph is rebuilt only when the capacity bucket, layout, placement or grid changes;
 // each call only rewrites the pinned descriptor buffer before the replay.
 #include <algorithm>
@@ -24,6 +27,7 @@ struct q1_batch_state {
   q1_device_aggregates* d_out = nullptr;
   q1_device_aggregates* d_spread = nullptr;  // spread_requests x kSpreadTables tables
   uint32_t spread_requests = 0;
+  q1_device_aggregates* h_out_dev = nullptr;  // h_out as the device addresses it (folds store there)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -97,7 +101,8 @@ static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placeme
   b->spread_requests = std::min(cap, kBatchSpreadRequests);
   const uint64_t spread_bytes = (uint64_t)b->spread_requests * kSpreadTables * sizeof(q1_device_aggregates);
   bool ok = hipHostMalloc(&b->h_buf, buf_bytes(cap), hipHostMallocDefault) == hipSuccess &&
-            hipHostMalloc(&b->h_out, cap * sizeof(q1_device_aggregates), hipHostMallocDefault) == hipSuccess &&
+            hipHostMalloc(&b->h_out, cap * sizeof(q1_device_aggregates), hipHostMallocMapped) == hipSuccess &&
+            hipHostGetDevicePointer((void**)&b->h_out_dev, b->h_out, 0) == hipSuccess &&
             hipMalloc(&b->d_buf, buf_bytes(cap)) == hipSuccess &&
             hipMalloc(&b->d_out, cap * sizeof(q1_device_aggregates)) == hipSuccess &&
             hipMalloc(&b->d_spread, spread_bytes) == hipSuccess &&
@@ -114,10 +119,12 @@ static q1_status build_state(q1_ctx* ctx, q1_layout layout, q1_placement placeme
   if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) st = Q1_EDEVICE;
   if (st == Q1_OK && hipMemcpyAsync(b->d_buf, b->h_buf, buf_bytes(cap), hipMemcpyHostToDevice, s) != hipSuccess)
     st = Q1_EDEVICE;
-  if (st == Q1_OK) st = launch_zero(b->d_out, cap, s);
+  const bool store = b->spread_requests == cap;  // every request has spread tables
+  if (st == Q1_OK && !store) st = launch_zero(b->d_out, cap, s);
   if (st == Q1_OK)
-    st = launch_batch(layout, placement, b->d_buf, b->d_out, b->d_spread, b->spread_requests, blocks, threads, s);
-  if (st == Q1_OK &&
+    st = launch_batch(layout, placement, b->d_buf, b->d_out, b->d_spread, b->spread_requests,
+                      store ? b->h_out_dev : nullptr, blocks, threads, s);
+  if (st == Q1_OK && !store &&
       hipMemcpyAsync(b->h_out, b->d_out, cap * sizeof(q1_device_aggregates), hipMemcpyDeviceToHost, s) != hipSuccess)
     st = Q1_EDEVICE;
   hipGraph_t g = nullptr;

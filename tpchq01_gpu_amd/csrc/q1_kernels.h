@@ -117,10 +117,12 @@ q1_placement batch_placement(q1_placement p);
 uint32_t batch_max_lane_rows(q1_placement p);
 uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p);
 // The batched kernel over the list in dev_buffer, then, with `spread` (spread_requests x
-// kSpreadTables tables, all zero), the fold of the first spread_requests requests' tables.
+// kSpreadTables tables, all zero), the fold of the first spread_requests requests' tables:
+// added into outs, or, with store_results (a device-visible pointer to the results in
+// pinned host memory, when every request has spread tables), stored there directly.
 q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* dev_buffer,
                        q1_device_aggregates* outs, q1_device_aggregates* spread, uint32_t spread_requests,
-                       uint32_t blocks, uint32_t threads, void* stream);
+                       q1_device_aggregates* store_results, uint32_t blocks, uint32_t threads, void* stream);
 // The host function launch_batch launches for (layout, placement), e.g. to find its node in
 // a captured graph.
 const void* batch_kernel_function(q1_layout layout, q1_placement placement);

@@ -25,8 +25,11 @@ __global__ void zero_words(uint64_t* p, uint64_t n) {
 static_assert(kSpreadTables == 64, "the fold gives each table one lane of a wavefront");
 static_assert(sizeof(q1_device_aggregates) == 48 * 8, "8 fields x 6 groups");
 // Block b folds tables[64 b .. 64 b + 63] into out[b] (one block for a scan's tables, one
-// per request for a batch's).
-__global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables, q1_device_aggregates* out) {
+// per request for a batch's).  STORE: out[b] receives the sums with plain stores (every
+// word, zeros included) instead of atomic adds, for a destination that holds nothing yet:
+// a batch's results in pinned host memory, which the stores reach over PCIe.
+template <bool STORE>
+__device__ __forceinline__ void fold_tables(q1_device_aggregates* tables, q1_device_aggregates* out) {
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   uint64_t* mine = reinterpret_cast<uint64_t*>(tables + (uint64_t)blockIdx.x * kSpreadTables + lane) + wave * 12;
   uint64_t* dst = reinterpret_cast<uint64_t*>(out + blockIdx.x) + wave * 12;
@@ -42,7 +45,11 @@ __global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables,
       const uint64_t sum = wave_sum_u64(v[i]);
       my = lane == (uint32_t)i ? sum : my;
     }
-    if (lane < 12 && my) atomicAdd((unsigned long long*)&dst[lane], (unsigned long long)my);
+    if (STORE) {
+      if (lane < 12) dst[lane] = my;
+    } else if (lane < 12 && my) {
+      atomicAdd((unsigned long long*)&dst[lane], (unsigned long long)my);
+    }
   } else {
     uint64_t my_lo = 0, my_hi = 0;
 #pragma unroll
@@ -58,10 +65,24 @@ __global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables,
       my_lo = lane == (uint32_t)g ? lo : my_lo;
       my_hi = lane == (uint32_t)g ? hi : my_hi;
     }
-    if (lane < (uint32_t)kNumGroups) atomic_add128(&dst[lane], &dst[6 + lane], my_lo, my_hi);
+    if (lane < (uint32_t)kNumGroups) {
+      if (STORE) {
+        dst[lane] = my_lo;
+        dst[6 + lane] = my_hi;
+      } else {
+        atomic_add128(&dst[lane], &dst[6 + lane], my_lo, my_hi);
+      }
+    }
   }
 #pragma unroll
   for (int i = 0; i < 12; i++) mine[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) fold_spread(q1_device_aggregates* tables, q1_device_aggregates* out) {
+  fold_tables<false>(tables, out);
+}
+__global__ void __launch_bounds__(256) fold_spread_store(q1_device_aggregates* tables, q1_device_aggregates* out) {
+  fold_tables<true>(tables, out);
 }
 
 // One thread per 32-row group writes the rows' values and the packed words of the group.
@@ -405,17 +426,22 @@ uint32_t batch_blocks_per_cu(q1_layout layout, q1_placement p) {
 }
 
 q1_status launch_batch(q1_layout layout, q1_placement placement, BatchHeader* buf, q1_device_aggregates* outs,
-                       q1_device_aggregates* spread, uint32_t spread_requests, uint32_t blocks, uint32_t threads,
-                       void* stream) {
+                       q1_device_aggregates* spread, uint32_t spread_requests, q1_device_aggregates* store_results,
+                       uint32_t blocks, uint32_t threads, void* stream) {
   placement = batch_placement(placement);
-  if (threads != (uint32_t)kBatchBlock || blocks == 0 || (spread_requests && !spread)) return Q1_EINVAL;
+  if (threads != (uint32_t)kBatchBlock || blocks == 0 || (spread_requests && !spread) ||
+      (store_results && !spread_requests))
+    return Q1_EINVAL;
   const BatchDesc* descs = reinterpret_cast<const BatchDesc*>(buf + 1);
   void* args[] = {(void*)&buf, (void*)&descs, (void*)&outs, (void*)&spread};
   const hipStream_t s = (hipStream_t)stream;
   q1_status st = hip_status(
       hipLaunchKernel(batch_kernel(layout, placement), dim3(blocks), dim3(threads), args, batch_lds(layout, placement), s));
   if (st == Q1_OK && spread_requests) {
-    hipLaunchKernelGGL(fold_spread, dim3(spread_requests), dim3(256), 0, s, spread, outs);
+    if (store_results)
+      hipLaunchKernelGGL(fold_spread_store, dim3(spread_requests), dim3(256), 0, s, spread, store_results);
+    else
+      hipLaunchKernelGGL(fold_spread, dim3(spread_requests), dim3(256), 0, s, spread, outs);
     st = hip_status(hipGetLastError());
   }
   return st;

@@ -6,7 +6,8 @@
 //   scheduler thread: whenever an in-flight slot is free, admits pending requests in
 //     arrival order (optionally holding them back batch_window_us for more while other
 //     batches are still running) and launches them as one batch on the slot's stream (a
-//     hipGraph per slot: descriptor copy, zero fill kernel, batched kernel, result copy).
+//     hipGraph per slot: descriptor copy, batched kernel, and the fold of the requests'
+//     spread tables, which stores the results into the slot's pinned host buffer).
 //   completion: a thread per slot waits for the slot's batch and publishes every result
 //     with its latency; a client waiting for a request of a launched batch waits for the
 //     device itself and publishes the batch first if it gets there first.
@@ -75,6 +76,7 @@ struct Slot {
   q1_device_aggregates* d_out = nullptr;
   q1_device_aggregates* d_spread = nullptr;  // the batch's per-request spread tables
   uint32_t spread_requests = 0;
+  q1_device_aggregates* h_out_dev = nullptr;  // h_out as the device addresses it
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraphNode_t kernel_node = nullptr;  // the batched kernel's node (its grid is set per batch)
@@ -123,6 +125,7 @@ void free_slot_buffers(Slot& s) {
   if (s.d_spread) (void)hipFree(s.d_spread);
   s.d_spread = nullptr;
   s.spread_requests = 0;
+  s.h_out_dev = nullptr;
   s.exec = nullptr;
   s.graph = nullptr;
   s.kernel_node = nullptr;
@@ -140,7 +143,8 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
   s.spread_requests = std::min(cap, q1::kBatchSpreadRequests);
   const uint64_t sb = (uint64_t)s.spread_requests * q1::kSpreadTables * sizeof(q1_device_aggregates);
   if (hipHostMalloc(&s.h_buf, bb, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc(&s.h_out, ob, hipHostMallocDefault) != hipSuccess || hipMalloc(&s.d_buf, bb) != hipSuccess ||
+      hipHostMalloc(&s.h_out, ob, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s.h_out_dev, s.h_out, 0) != hipSuccess || hipMalloc(&s.d_buf, bb) != hipSuccess ||
       hipMalloc(&s.d_out, ob) != hipSuccess || hipMalloc(&s.d_spread, sb) != hipSuccess) {
     free_slot_buffers(s);
     return Q1_ENOMEM;
@@ -155,11 +159,13 @@ q1_status prepare_slot(q1_server* sv, Slot& s, uint32_t cap, q1_layout layout) {
   q1_status st = Q1_OK;
   if (hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) st = Q1_EDEVICE;
   if (st == Q1_OK && hipMemcpyAsync(s.d_buf, s.h_buf, bb, hipMemcpyHostToDevice, s.stream) != hipSuccess) st = Q1_EDEVICE;
-  if (st == Q1_OK) st = q1::launch_zero(s.d_out, cap, s.stream);
+  // With spread tables for every request the fold stores the results into h_out itself.
+  const bool store = s.spread_requests == cap;
+  if (st == Q1_OK && !store) st = q1::launch_zero(s.d_out, cap, s.stream);
   if (st == Q1_OK)
-    st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, s.d_spread, s.spread_requests, blocks,
-                          q1::kBatchThreads, s.stream);
-  if (st == Q1_OK && hipMemcpyAsync(s.h_out, s.d_out, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
+    st = q1::launch_batch(layout, sv->placement, s.d_buf, s.d_out, s.d_spread, s.spread_requests,
+                          store ? s.h_out_dev : nullptr, blocks, q1::kBatchThreads, s.stream);
+  if (st == Q1_OK && !store && hipMemcpyAsync(s.h_out, s.d_out, ob, hipMemcpyDeviceToHost, s.stream) != hipSuccess)
     st = Q1_EDEVICE;
   hipGraph_t g = nullptr;
   const hipError_t ec = hipStreamEndCapture(s.stream, &g);
