@@ -619,10 +619,6 @@ def main():
         log(f"bench: {args.cpu_rows} uncompressed rows through every placement")
         out["uncompressed_60M"] = uncompressed_leg(q1, ctx, args.cpu_rows, 0x51C00000)
         log(f"bench: uncompressed_60M {out['uncompressed_60M']['placements']}")
-    if args.host and layout == q1.COMPRESSED:
-        log("bench: host-resident leg (streaming, CPU/GPU split)")
-        out["host_resident"] = host_leg(q1, ctx, table, rows, 0x51C00002, one, args.cpu_threads)
-        log(f"bench: host_resident {out['host_resident']}")
     spec, batch_results = [], None
     if args.batch_requests:
         from tpchq01_gpu_amd import workloads
@@ -636,6 +632,12 @@ def main():
         out["batch"]["resident_read_ceiling_gbps"] = res
         out["batch"]["frac_of_resident_read_ceiling"] = out["batch"]["batched_device_gbps"] / res if res else None
         log(f"bench: batch {out['batch']}")
+    # after the batch leg: the batch leg's one-at-a-time loop of q1_run calls measured 4x slower
+    # when it ran after the host-resident leg
+    if args.host and layout == q1.COMPRESSED:
+        log("bench: host-resident leg (streaming, CPU/GPU split)")
+        out["host_resident"] = host_leg(q1, ctx, table, rows, 0x51C00002, one, args.cpu_threads)
+        log(f"bench: host_resident {out['host_resident']}")
     plan = [(float(r), float(s)) for r, s in (x.split(":") for x in args.serve_plan.split(",") if x)]
     if plan:
         log(f"bench: serving leg, {plan} (req/s : seconds)")

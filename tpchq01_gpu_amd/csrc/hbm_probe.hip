@@ -88,23 +88,18 @@ __global__ __launch_bounds__(B) void read_columns_nt(Columns c, uint64_t tiles4,
 }
 
 // The batched kernel's structure with the same loads and no arithmetic: a resident grid
-// works through a list of entries of B lanes x tiles_per_entry 4-row tiles, block b first
-// running entry b and then claiming entries gridDim.x, gridDim.x + 1, ... from `counter`
-// (0 at launch) in the order blocks finish.  Every block exits once the claims pass the
-// last entry.
+// works through a list of entries of B lanes x tiles_per_entry 4-row tiles with a fixed
+// stride, block b running entries b, b + gridDim.x, b + 2 gridDim.x, ... and waiting at a
+// barrier after each entry (where the batched kernel merges).
 template <int B>
 __global__ __launch_bounds__(B) void read_columns_resident(Columns c, uint64_t tiles4, uint32_t tiles_per_entry,
-                                                           uint32_t* counter, uint32_t* sink) {
-  __shared__ uint32_t next;
+                                                           uint32_t* sink) {
   const uint64_t per_entry = (uint64_t)B * tiles_per_entry;
   const uint64_t entries = (tiles4 + per_entry - 1) / per_entry;
   const uint8_t* rf = reinterpret_cast<const uint8_t*>(c.return_flag);
   const uint8_t* ls = reinterpret_cast<const uint8_t*>(c.line_status);
   uint32_t acc = 0;
-  uint64_t v = blockIdx.x;
-  while (v < entries) {
-    uint32_t claim = 0;
-    if (threadIdx.x == 0) claim = atomicInc(counter, 0xFFFFFFFFu);
+  for (uint64_t v = blockIdx.x; v < entries; v += gridDim.x) {
     const uint64_t begin = v * per_entry;
     const uint64_t end = begin + per_entry < tiles4 ? begin + per_entry : tiles4;
     for (uint64_t t = begin + threadIdx.x; t < end; t += B) {
@@ -115,10 +110,7 @@ __global__ __launch_bounds__(B) void read_columns_resident(Columns c, uint64_t t
       const u32x4_t p = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(c.price) + t);
       acc += s.x ^ s.y ^ q ^ d ^ x ^ p.x ^ p.y ^ p.z ^ p.w ^ rf[t] ^ (ls[t >> 1] >> (4 * (t & 1)));
     }
-    if (threadIdx.x == 0) next = gridDim.x + claim;
     __syncthreads();
-    v = (uint32_t)__builtin_amdgcn_readfirstlane((int)next);
-    __syncthreads();  // every lane has read `next` before thread 0 writes it again
   }
   if (acc == kSentinel) sink[0] = acc;
 }
@@ -318,20 +310,16 @@ int main(int argc, char** argv) {
     std::printf("%s{\"threads_per_block\": %d, \"tuples_per_thread\": %d, \"ms\": %.4f, \"gbps\": %.1f}", s ? ", " : "",
                 b, stream_shapes[s][1], ms, bytes / ms / 1e6);
   }
-  // the batched kernel's tuned structure: 8 resident blocks per compute unit x 256 lanes,
-  // entries of 192 rows per lane claimed one at a time
+  // the batched kernel's tuned structure: 8 blocks per compute unit x 256 lanes, entries of
+  // 192 rows per lane taken with a fixed stride
   int cus = 0;
   CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
-  uint32_t* counter = nullptr;
-  CHECK(hipMalloc(&counter, 4));
   const uint32_t res_blocks = 8u * (uint32_t)cus, res_tiles = 192 / 4;
   const double res_ms = back_to_back_ms([&] {
-    CHECK(hipMemsetAsync(counter, 0, 4));
-    read_columns_resident<256><<<dim3(res_blocks), dim3(256)>>>(c, tiles4, res_tiles, counter, sink);
+    read_columns_resident<256><<<dim3(res_blocks), dim3(256)>>>(c, tiles4, res_tiles, sink);
   }, reps);
   std::printf("], \"resident_streaming\": {\"blocks\": %u, \"threads_per_block\": 256, \"rows_per_lane\": 192, "
               "\"ms\": %.4f, \"gbps\": %.1f}", res_blocks, res_ms, bytes / res_ms / 1e6);
-  CHECK(hipFree(counter));
   std::printf(", \"flat\": [");
   const uint64_t n16 = bytes / 16;
   const int flat[4][2] = {{64, 64}, {256, 64}, {256, 256}, {512, 64}};
