@@ -195,9 +195,11 @@ def batch_leg(q1, ctx, spec, headline=None):
     tables = [ctx.generate(q1.COMPRESSED, spec[r][0], int(sizes[r])) for r in range(n)]
     reqs = (q1.Request * n)(*[q1.Request(tables[r].handle, 0, int(sizes[r]), int(cutoffs[r]), 0) for r in range(n)])
     outs = (q1.Result * n)()
-    for _ in range(2):   # build the graph, then warm
+    # Build the graph, then warm: over the first ten or so back-to-back batches the device
+    # time rises and falls again by 4-6 %, whatever ran before.
+    for _ in range(8):
         ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
-    reps = 5
+    reps = 20
     dev = []
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -236,6 +238,7 @@ def batch_leg(q1, ctx, spec, headline=None):
         t.free()
     rows = int(sizes.sum())
     return {"requests": n, "rows": rows, "batched_ms": batched * 1e3, "batched_device_ms": batched_dev * 1e3,
+            "batched_device_ms_range": [min(dev), max(dev)],
             "batched_req_s": n / batched, "batched_rows_s": rows / batched,
             "batched_device_gbps": rows * q1.BITS_PER_ROW[q1.COMPRESSED] / 8 / batched_dev / 1e9,
             "one_at_a_time_ms": one * 1e3, "one_at_a_time_req_s": n / one,
@@ -576,6 +579,22 @@ def main():
                      "frac_of_read_ceiling": achieved / ceiling if ceiling else None,
                      "copy_peak_gbps": probe.get("copy_gbps") if probe else None},
     }
+    # The batch leg runs before the legs that allocate and free tens of GB (layouts,
+    # placements, host-resident): after the host-resident leg its one-at-a-time loop of
+    # q1_run calls measured up to 4x slower.
+    spec, batch_results = [], None
+    if args.batch_requests:
+        from tpchq01_gpu_amd import workloads
+        spec = workloads.batch_spec(args.batch_requests)
+        log(f"bench: batch leg, {args.batch_requests} requests")
+        out["batch"], batch_results = batch_leg(q1, ctx, spec,
+                                                (table, rows) if layout == q1.COMPRESSED else None)
+        # the same columns read by the batched kernel's resident-grid structure with no
+        # arithmetic (hbm_probe's resident pattern): the batch's own ceiling
+        res = probe.get("resident_gbps") if probe else None
+        out["batch"]["resident_read_ceiling_gbps"] = res
+        out["batch"]["frac_of_resident_read_ceiling"] = out["batch"]["batched_device_gbps"] / res if res else None
+        log(f"bench: batch {out['batch']}")
     if args.layouts and layout == q1.COMPRESSED:
         log("bench: layouts leg")
         out["layouts"] = layouts_leg(q1, ctx, table, rows, 0x51C00002, probe=probe)
@@ -619,21 +638,6 @@ def main():
         log(f"bench: {args.cpu_rows} uncompressed rows through every placement")
         out["uncompressed_60M"] = uncompressed_leg(q1, ctx, args.cpu_rows, 0x51C00000)
         log(f"bench: uncompressed_60M {out['uncompressed_60M']['placements']}")
-    spec, batch_results = [], None
-    if args.batch_requests:
-        from tpchq01_gpu_amd import workloads
-        spec = workloads.batch_spec(args.batch_requests)
-        log(f"bench: batch leg, {args.batch_requests} requests")
-        out["batch"], batch_results = batch_leg(q1, ctx, spec,
-                                                (table, rows) if layout == q1.COMPRESSED else None)
-        # the same columns read by the batched kernel's resident-grid structure with no
-        # arithmetic (hbm_probe's resident pattern): the batch's own ceiling
-        res = probe.get("resident_gbps") if probe else None
-        out["batch"]["resident_read_ceiling_gbps"] = res
-        out["batch"]["frac_of_resident_read_ceiling"] = out["batch"]["batched_device_gbps"] / res if res else None
-        log(f"bench: batch {out['batch']}")
-    # after the batch leg: the batch leg's one-at-a-time loop of q1_run calls measured 4x slower
-    # when it ran after the host-resident leg
     if args.host and layout == q1.COMPRESSED:
         log("bench: host-resident leg (streaming, CPU/GPU split)")
         out["host_resident"] = host_leg(q1, ctx, table, rows, 0x51C00002, one, args.cpu_threads)
