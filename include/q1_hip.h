@@ -251,8 +251,10 @@ typedef struct {
  * share one launch; any other mix runs request by request.  `launch` may be NULL; its
  * threads_per_block must be 0 or 256; tuples_per_thread (rows per lane of each block of a
  * request; 0 = the tuned value) a multiple of 16, at most 256 for the shared placement and
- * 16384 for the register one; num_blocks 0 = as many blocks as are resident at once
- * (below 2^24). */
+ * 16384 for the register one; num_blocks (the resident grid, below 2^24) 0 = the tuned
+ * blocks per compute unit x compute units.  The context keeps the batch's buffers between
+ * calls of one capacity (the next power of two >= n, at least 16): the descriptors, the
+ * results and 24 KiB of partial tables per request for the first 1024 requests. */
 q1_status q1_run_batch(q1_ctx* ctx, const q1_request* reqs, uint32_t n, const q1_launch* launch,
                        q1_result* outs);
 
