@@ -362,6 +362,30 @@ def test_batch_small_grids_span_many_requests(ctx):
     t.free()
 
 
+def test_batch_beyond_the_spread_requests(ctx):
+    """A batch of 1500 requests: the first 1024 merge through their spread tables, the rest
+    straight into their results (the graph then zero-fills and copies the device results).
+    Requests of both kinds share entries' neighbourhoods, some are empty, one is large;
+    every result equals the request run alone, on two calls in a row (the spread tables
+    must come back clean from the first)."""
+    rng = np.random.default_rng(SEED + 40)
+    t = ctx.generate(q1.COMPRESSED, SEED + 41, 3_000_017)
+    n = 1500
+    sizes = rng.integers(0, 40_000, n)
+    sizes[[0, 700, 1023, 1024, 1499]] = [0, 0, 2_500_000, 0, 1]
+    begins = [int(rng.integers(0, t.rows - s + 1)) for s in sizes]
+    cutoffs = [729999 - int(rng.integers(0, 120)) for _ in range(n)]
+    reqs = (q1.Request * n)(*[q1.Request(t.handle, b, int(s), c, 0) for b, s, c in zip(begins, sizes, cutoffs)])
+    want = [ctx.run(t, b, int(s), c, q1.IN_REGISTERS) for b, s, c in zip(begins, sizes, cutoffs)]
+    for call in range(2):
+        outs = (q1.Result * n)()
+        note(f"case batch of {n} requests, call {call}")
+        ctx.check(q1.lib.q1_run_batch(ctx.handle, reqs, n, None, outs), "q1_run_batch")
+        for i in range(n):
+            assert q1.result_to_list(outs[i]) == want[i], (call, i, begins[i], int(sizes[i]))
+    t.free()
+
+
 def test_batch_rows_per_lane_and_empty_requests(ctx):
     """The batched kernel cuts each request into blocks of 256 lanes x tuples_per_thread
     rows.  Small values give large requests many blocks; empty requests (first, between
